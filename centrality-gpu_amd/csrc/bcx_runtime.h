@@ -1,0 +1,118 @@
+// Runtime types shared by the plain and batched host drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "bcx.h"
+#include "bcx_device.h"
+#include "bcx_kernels.h"
+
+struct bcx_ctx;
+
+namespace bcx {
+
+// Grow-only device buffer accounted against the context's memory cap.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(bcx_ctx *ctx, size_t need, const char *what);
+    void release(bcx_ctx *ctx);
+};
+
+struct Timer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    size_t used = 0;
+    void begin(bcx_ctx *ctx);
+    void end(bcx_ctx *ctx);
+    double collect();
+    void destroy();
+};
+
+// Host copy of a component (planning needs the chains and the degrees).
+struct HostComp {
+    int32_t n = 0;
+    int64_t arcs = 0;
+    std::vector<int32_t> rp, col, reach;
+    int32_t n_chains = 0;
+    std::vector<int32_t> ch_ptr, ch_nodes, ch_ends;
+};
+
+struct DComp {
+    int32_t n = 0;
+    int64_t arcs = 0;
+    int64_t offset = 0;  // into the concatenated BC vector
+    HostComp host;
+    DBuf b_rp, b_col, b_reach, b_chp, b_chn, b_che;
+    CompDev dev{};
+    ChainDev chains{};
+};
+
+// One endpoint-row batch: BFS work (endpoints, with slot and mode) then the
+// chain sources that combine those rows.
+struct PlanBatch {
+    std::vector<int32_t> bsrc, bslot;
+    std::vector<int8_t> bmode;
+    std::vector<int32_t> cci, cpos, cslots;
+    int32_t n_rows = 0;
+};
+
+struct Plan {
+    std::vector<int32_t> free_src;     // BFS sources that need no row
+    std::vector<PlanBatch> batches;
+    int64_t n_bfs = 0, n_chain = 0, n_row_only = 0;
+};
+
+int make_plan(const HostComp &hc, const int32_t *subset, int64_t n_sub, bool use_chains,
+              int64_t row_cap, Plan &plan);
+
+}  // namespace bcx
+
+struct BatchedState;  // batched path buffers (bcx_batched_host.cpp)
+
+struct bcx_ctx {
+    int device = 0;
+    int n_cu = 0;
+    std::string name;
+    size_t mem_cap = 0, mem_used = 0;
+    std::map<void *, size_t> sizes;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool overflow = false;
+    int plain_wg_per_cu = 2;
+    bcx::Timer timer, timer_chain;
+    bcx::DBuf counters, bc_acc, plain_scratch, rows_buf;
+    bcx::DBuf w_src, w_aux, w_mode, w_slots;
+    BatchedState *batched = nullptr;
+    std::vector<bcx::DBuf *> all_bufs() {
+        return {&counters, &bc_acc, &plain_scratch, &rows_buf, &w_src, &w_aux, &w_mode, &w_slots};
+    }
+};
+
+struct bcx_dgraph {
+    std::vector<bcx::DComp> comps;
+    int64_t total_n = 0;
+};
+
+// shared helpers (bcx_runtime.hip)
+int ctx_fail(bcx_ctx *ctx, int code, const std::string &msg);
+int ctx_hip(bcx_ctx *ctx, hipError_t e, const char *what);
+int dev_alloc(bcx_ctx *ctx, void **p, size_t bytes, const char *what);
+void dev_free(bcx_ctx *ctx, void *p);
+int upload(bcx_ctx *ctx, const void *host, size_t bytes, bcx::DBuf &buf, const char *what);
+int read_counters(bcx_ctx *ctx, bcx_stats *st);
+int64_t row_capacity(bcx_ctx *ctx, int32_t n, const bcx_opts *o);
+int plain_component(bcx_ctx *ctx, const bcx::DComp &dc, const bcx::Plan &plan, const bcx_opts *o,
+                    double *bc, bcx_stats *st);
+
+// batched path (bcx_batched_host.cpp)
+int batched_component(bcx_ctx *ctx, bcx_dgraph *g, bcx::DComp &dc, const bcx::Plan &plan,
+                      const bcx_opts *o, double *bc, bcx_stats *st);
+int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o,
+                         int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta,
+                         int32_t *sigma_is_f64);
+void batched_release(bcx_ctx *ctx);
+void batched_graph_release(bcx_ctx *ctx, bcx_dgraph *g);

@@ -1,0 +1,217 @@
+/*
+ * bcx.h - C-ABI of the MI355X exact betweenness-centrality (BC) hot path.
+ *
+ * Two halves, both plain C (no torch or STL types):
+ *
+ *  1. Host structure (bcx_host_*): edge-list reader, biconnected-component
+ *     split in Tarjan order, per-component local relabel, block-cut-tree reach
+ *     weights and cut-vertex term, degree-2 chain extraction.  These replace
+ *     the host C++ of the reference's L4 layer.
+ *
+ *  2. GPU engine (bcx_create / bcx_upload / bcx_run / bcx_bc ...): per-source
+ *     Brandes forward (levels, sigma) and backward (dependency) kernels on
+ *     gfx950, chain-source combine, weighted BC accumulation.  These replace
+ *     the reference's host->GPU boundary begin_gpu / bc_gpu_free /
+ *     bc_gpu_active / postprocess.
+ *
+ * Every entry point returns a status code (BCX_OK == 0) and never exits the
+ * process; bcx_last_error() / bcx_host_last_error() give the message.
+ *
+ * BC convention: ordered pairs, not halved, i.e.
+ *   BC(v) = sum_{s != v != t} sigma_st(v) / sigma_st
+ *         = 2 x networkx.betweenness_centrality(G, normalized=False)
+ * (reference: Codes/kernels_free.cu:423-426, Codes/modified_apsp.cpp:209).
+ */
+#ifndef BCX_H_
+#define BCX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BCX_OK             0
+#define BCX_E_ARG         -1  /* bad argument / shape                        */
+#define BCX_E_IO          -2  /* input file missing or malformed             */
+#define BCX_E_GRAPH       -3  /* self loop, duplicate edge, disconnected ...  */
+#define BCX_E_NOMEM       -4  /* device memory cap exceeded                   */
+#define BCX_E_HIP         -5  /* HIP runtime error                            */
+#define BCX_E_NODEV       -6  /* no GPU / library built without a device      */
+#define BCX_E_OVERFLOW    -7  /* sigma overflow that could not be recovered   */
+
+/* ======================================================================= */
+/* 1. Host structure                                                        */
+/* ======================================================================= */
+
+typedef struct bcx_host bcx_host;
+
+/* One biconnected component with more than two vertices, in local ids.
+ * Local ids are assigned in first-appearance order over the component's edge
+ * list in Tarjan pop order (Codes/modified_apsp.cpp:336-357); the adjacency
+ * of every local vertex lists its arcs in the order (u->v),(v->u) of that edge
+ * list (Codes/to_eargraph_construct.cpp:81-96, Codes/graph_ear_decompos.h:65-83).
+ *
+ * Chains are the maximal paths of component-degree-2 vertices between two
+ * component-degree>=3 endpoints (what Codes/bcc_bfs.cpp:100-236 emits as
+ * ear_premble{lnode, rnode, cnt, ld[], rd[], nodes[]}, Codes/ear_graph.h:3-32).
+ * Chain i has endpoints chain_ends[2i] (= L) and chain_ends[2i+1] (= R) and
+ * vertices chain_nodes[chain_ptr[i] .. chain_ptr[i+1]) listed from the L side;
+ * the j-th of them (0-based) has ld = j+1 and rd = cnt-j.                   */
+typedef struct {
+    int32_t        n;            /* local vertices                           */
+    int64_t        arcs;         /* 2 x undirected edges                     */
+    const int32_t *row_ptr;      /* [n+1] (arcs < 2^31)                      */
+    const int32_t *col;          /* [arcs]                                   */
+    const int32_t *reach;        /* [n]  r_B(v) >= 0                         */
+    int32_t        n_chains;     /* 0 = no chain reduction                   */
+    const int32_t *chain_ptr;    /* [n_chains+1]                             */
+    const int32_t *chain_nodes;  /* [chain_ptr[n_chains]]                    */
+    const int32_t *chain_ends;   /* [2*n_chains]                             */
+} bcx_component;
+
+/* Graph::Graph(char*) (Codes/graph.cpp:38-72): "n m" header then m lines
+ * "u v w", 0-based, w == 1.  Adjacency keeps file order.                  */
+int  bcx_host_from_file(const char *path, bcx_host **out);
+/* Same graph from an in-memory edge list uv[2*m] (file order).            */
+int  bcx_host_from_edges(int32_t n, int64_t m, const int32_t *uv, bcx_host **out);
+void bcx_host_free(bcx_host *h);
+const char *bcx_host_last_error(const bcx_host *h);
+
+int32_t bcx_host_num_vertices(const bcx_host *h);
+int64_t bcx_host_num_edges(const bcx_host *h);
+
+/* Run the structural reduction: Tarjan BCC split (Codes/bicc.cpp:39-161),
+ * local relabel (Codes/modified_apsp.cpp:266-363), block-cut tree reach and
+ * cut term (Codes/modified_apsp.cpp:77-220, restated as a closed form),
+ * chains (Codes/bcc_bfs.cpp:100-236, 928-1311) when use_chains != 0.
+ * check_input != 0 rejects self loops, duplicate edges and disconnected
+ * input with BCX_E_GRAPH.                                                  */
+int  bcx_host_decompose(bcx_host *h, int use_chains, int check_input);
+
+/* Number of BCCs of any size, in Tarjan emission order (" n_bcc= ").      */
+int32_t bcx_host_num_bcc(const bcx_host *h);
+/* Size of BCC i: vertices and undirected edges.                            */
+int  bcx_host_bcc_size(const bcx_host *h, int32_t i, int32_t *n, int64_t *edges);
+/* Edge list of BCC i in pop order, global ids, uv[2*edges].                */
+int  bcx_host_bcc_edges(const bcx_host *h, int32_t i, int32_t *uv);
+/* Articulation points: flags[n] (1 = articulation point).                  */
+int  bcx_host_articulation(const bcx_host *h, uint8_t *flags);
+/* Cut-vertex term (ordered pairs separated by v), int64[n].                */
+int  bcx_host_cut_term(const bcx_host *h, int64_t *cut);
+
+/* Components handed to the GPU: the BCCs with more than two vertices, in
+ * Tarjan order (Codes/to_eargraph_construct.cpp:54-60).                    */
+int32_t bcx_host_num_components(const bcx_host *h);
+/* Views into host-owned arrays; valid until bcx_host_free.                 */
+int  bcx_host_component(const bcx_host *h, int32_t c, bcx_component *out,
+                        const int32_t **global_ids, int32_t *bcc_index);
+/* Per-component counts as printed by the reference: free sources (not chain
+ * vertex, not chain endpoint) and distinct chain endpoints.                */
+int  bcx_host_component_counts(const bcx_host *h, int32_t c,
+                               int32_t *n_free, int32_t *n_endpoints,
+                               int32_t *n_chain_vertices);
+
+/* ======================================================================= */
+/* 2. GPU engine                                                            */
+/* ======================================================================= */
+
+typedef struct bcx_ctx    bcx_ctx;
+typedef struct bcx_dgraph bcx_dgraph;   /* device-resident component set */
+
+/* Precision of the dependency (delta) arithmetic.                          */
+#define BCX_PREC_FP64        0   /* FP64 delta, FP64 BC                       */
+#define BCX_PREC_FP32C       1   /* FP32 compensated (Kahan) delta, FP64 BC   */
+
+/* Schedules.                                                               */
+#define BCX_PATH_PLAIN       0   /* per-component launches, block per source */
+#define BCX_PATH_BATCHED     1   /* multi-source batched persistent path     */
+
+typedef struct {
+    int32_t        path;          /* BCX_PATH_*                              */
+    int32_t        precision;     /* BCX_PREC_*                              */
+    int32_t        use_chains;    /* 1: chain vertices via endpoint rows     */
+    int32_t        batch_width;   /* batched: sources per tile (0 = auto)    */
+    const int64_t *src_subset;    /* NULL = all sources; else slots          */
+    int64_t        n_src;         /* slot = comp_offset[c] + local vertex    */
+    int64_t        row_cap_bytes; /* endpoint-row slab cap (0 = auto)        */
+    int32_t        sigma_f64_threshold_log2; /* test hook: treat sigma >=
+                                     2^k as overflow (0 = 64, i.e. uint64)  */
+    int32_t        flags;         /* reserved, 0                              */
+} bcx_opts;
+
+typedef struct {
+    double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */
+    double  t_total_s;         /* wall time of bcx_run                        */
+    int64_t n_bfs_sources;     /* forward+backward sources                    */
+    int64_t n_chain_sources;   /* combine+backward sources                    */
+    int64_t n_row_only;        /* extra forward-only endpoint rows            */
+    int64_t n_f64_sigma;       /* sources redone with FP64 sigma              */
+    int64_t n_launches;        /* kernel launches                             */
+    int32_t max_level;         /* deepest BFS level seen                      */
+    int32_t reserved;
+    double  bytes_alg;         /* algorithmic HBM bytes of the work done      */
+    double  teps_work;         /* sum over sources of component arcs/2        */
+    double  t_chain_s;         /* part of t_kernel_s spent on chain sources   */
+} bcx_stats;
+
+/* Context: device, streams, scratch.  mem_cap (bytes, 0 = 90% of free
+ * memory) bounds every device allocation the context makes; exceeding it
+ * returns BCX_E_NOMEM (the reference exits, Codes/postprocesses.cu:58-62). */
+int  bcx_create(bcx_ctx **out, int device, size_t mem_cap);
+void bcx_destroy(bcx_ctx *ctx);
+const char *bcx_last_error(const bcx_ctx *ctx);
+/* Number of compute units and the device name, for reports.               */
+int  bcx_device_info(bcx_ctx *ctx, int32_t *n_cu, char *name, int32_t name_len);
+
+/* Upload components once (replaces the per-BCC cudaMemcpy of
+ * Codes/kernels_free.cu:43-51 and Codes/kernels_active.cu:69-78).         */
+int  bcx_upload(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps,
+                bcx_dgraph **out);
+void bcx_dgraph_free(bcx_ctx *ctx, bcx_dgraph *g);
+
+/* Weighted Brandes over every component of g (begin_gpu free + active +
+ * postprocess, Codes/begin_gpupgm.cu:42-89).  bc_out is a host array of
+ * sum(n_c) doubles, component c at offset sum_{c'<c} n_{c'}; results are
+ * ADDED to it.  Per-source weight (r(s)+1), per-target weight (r(t)+1)
+ * (Codes/kernels_free.cu:391,409,425).                                      */
+int  bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts,
+             double *bc_out, bcx_stats *st);
+
+/* Convenience: upload + run + free (one call per component set).          */
+int  bcx_bc(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps,
+            const bcx_opts *opts, double *bc_out, bcx_stats *st);
+
+/* Whole pipeline for a decomposed host graph: GPU BC of every component
+ * scattered to global ids plus the cut-vertex term (the reference's
+ * bc_result, Codes/to_eargraph_construct.cpp:166-179 without the double
+ * addition of the free-source BC).  bc_out[n] is overwritten.             */
+int  bcx_host_bc(bcx_ctx *ctx, const bcx_host *h, const bcx_opts *opts,
+                 double *bc_out, bcx_stats *st);
+
+/* Parity probe: levels, sigma and delta of one source of one component, as
+ * the chosen path computes them (chain vertices go through the chain-source
+ * combine when opts->use_chains).  Any output pointer may be NULL.  sigma is
+ * reported in both forms; sigma_is_f64 tells which one is authoritative.   */
+int  bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src,
+                      const bcx_opts *opts, int32_t *level, uint64_t *sigma_u64,
+                      double *sigma_f64, double *delta, int32_t *sigma_is_f64);
+
+/* ======================================================================= */
+/* 3. Synthetic workloads                                                    */
+/* ======================================================================= */
+
+/* Seeded generators of the five benchmark workloads (1..5) plus small
+ * variants; writes an edge list in file order.  scale in (0,1] shrinks the
+ * vertex budget (1 = full size).  *uv is allocated with malloc and must be
+ * released with bcx_free.                                                  */
+int  bcx_generate(int32_t config, uint64_t seed, double scale,
+                  int32_t *n, int64_t *m, int32_t **uv);
+void bcx_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BCX_H_ */

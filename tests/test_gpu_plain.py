@@ -1,0 +1,183 @@
+"""GPU parity of the plain per-source path through the C-ABI.
+
+Bar: levels and sigma bit-exact against the oracle (uint64 sigma, and FP64
+sigma summed in the oracle's CSR pull order); delta within 1e-12 relative;
+BC within 1e-9 relative (FP64 atomics reorder sums), zero-BC vertices exact.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import bcx
+import oracle
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+
+BC_TOL = 1e-9
+
+
+def comps_of(n, uv, chains=True):
+    return bcx.Host.from_edges(n, uv).decompose(use_chains=chains).components()
+
+
+def check_source(engine, c, s, **kw):
+    lv, su, sf, dl, isf = engine.debug_source(c, s, **kw)
+    olv, osu, osf, odl, ovf = oracle.source(c.row_ptr, c.col, c.reach, s)
+    np.testing.assert_array_equal(lv, olv)
+    if isf:
+        assert ovf or kw.get("sigma_threshold_log2", 0)
+        np.testing.assert_array_equal(sf, osf)  # bit-exact FP64 pull order
+    else:
+        assert not ovf
+        np.testing.assert_array_equal(su, osu)
+    np.testing.assert_allclose(dl, odl, rtol=1e-12, atol=1e-300)
+
+
+def test_levels_sigma_delta_goldens(engine, goldens):
+    for name, g in goldens.items():
+        print("case", name, flush=True)
+        for c in comps_of(g["n"], g["edges"], chains=False):
+            for s in range(c.n):
+                check_source(engine, c, s, use_chains=False)
+
+
+def test_chain_sources_match_direct(engine, goldens):
+    for name in ("sample_test1", "sample_test2", "sample_test4", "random0", "random1", "random2"):
+        g = goldens[name]
+        print("case", name, flush=True)
+        for c in comps_of(g["n"], g["edges"]):
+            for k in range(c.n_chains):
+                for x in c.chain_nodes[c.chain_ptr[k]:c.chain_ptr[k + 1]]:
+                    check_source(engine, c, int(x), use_chains=True)
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 0.25), (2, 0.003), (4, 0.05), (5, 0.003)])
+def test_sampled_sources_generated(engine, cfg, scale):
+    n, uv = bcx.generate(cfg, 0, scale)
+    comps = comps_of(n, uv)
+    big = max(comps, key=lambda c: c.n)
+    rng = np.random.default_rng(cfg)
+    print("case cfg", cfg, "n", big.n, flush=True)
+    for s in rng.choice(big.n, size=min(24, big.n), replace=False):
+        check_source(engine, big, int(s), use_chains=True)
+
+
+@pytest.mark.parametrize("chains", [False, True])
+def test_component_bc_goldens(engine, goldens, chains):
+    for name, g in goldens.items():
+        comps = comps_of(g["n"], g["edges"], chains)
+        if not comps:
+            continue
+        got, st = engine.bc(comps, use_chains=chains)
+        for c, b in zip(comps, got):
+            want = oracle.component_bc(c.row_ptr, c.col, c.reach)
+            mx, mean, zeros = relerr(b, want)
+            assert mx < BC_TOL and zeros, name
+
+
+def test_whole_graph_bc_goldens(engine, goldens):
+    for name, g in goldens.items():
+        h = bcx.Host.from_edges(g["n"], g["edges"]).decompose()
+        bc, st = engine.host_bc(h, use_chains=True)
+        mx, mean, zeros = relerr(bc, g["bc"])
+        assert mx < BC_TOL and zeros, name
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 1.0), (2, 0.003), (4, 0.1), (5, 0.003)])
+def test_whole_graph_bc_generated(engine, cfg, scale):
+    n, uv = bcx.generate(cfg, 0, scale)
+    h = bcx.Host.from_edges(n, uv).decompose()
+    rp, col = oracle.csr_from_edges(n, uv)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    print("case cfg", cfg, "n", n, flush=True)
+    bc, st = engine.host_bc(h, use_chains=True)
+    mx, mean, zeros = relerr(bc, want)
+    print("maxrel", mx, "mean", mean, st, flush=True)
+    assert mx < BC_TOL and zeros
+
+
+def test_fp64_sigma_grid(engine):
+    """40x40 grid: corner sigma C(78,39) > 2^64 -> FP64 sigma, bit-exact pull order."""
+    w = 40
+    edges = []
+    for y in range(w):
+        for x in range(w):
+            v = y * w + x
+            if x + 1 < w:
+                edges.append((v, v + 1))
+            if y + 1 < w:
+                edges.append((v, v + w))
+    comps = comps_of(w * w, edges)
+    c = comps[0]
+    for s in (0, w - 1, w * w // 2 + 3):
+        check_source(engine, c, s, use_chains=False)
+    got, st = engine.bc(comps, use_chains=True)
+    assert st["n_f64_sigma"] > 0
+    mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
+    assert mx < BC_TOL and zeros
+
+
+def test_sigma_threshold_hook_forces_fp64(engine):
+    n, uv = bcx.generate(4, 0, 0.05)
+    comps = comps_of(n, uv)
+    c = max(comps, key=lambda c: c.n)
+    check_source(engine, c, 0, use_chains=False, sigma_threshold_log2=6)
+    got, st = engine.bc([c], use_chains=True, sigma_threshold_log2=6)
+    assert st["n_f64_sigma"] > 0
+    mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
+    assert mx < BC_TOL and zeros
+
+
+def test_subset_and_tiny_row_cap(engine):
+    """Source subsets and a 2-row endpoint slab (many chain batches, rows recomputed)."""
+    n, uv = bcx.generate(1, 0, 0.5)
+    comps = comps_of(n, uv)
+    c = max(comps, key=lambda c: c.n)
+    g = engine.upload([c])
+    try:
+        rng = np.random.default_rng(11)
+        sub = np.sort(rng.choice(c.n, size=c.n // 3, replace=False))
+        got, st = g.run(use_chains=True, subset=sub)
+        want = oracle.component_bc(c.row_ptr, c.col, c.reach, sources=sub)
+        mx, _, zeros = relerr(got, want)
+        assert mx < BC_TOL and zeros
+        got, st = g.run(use_chains=True, row_cap_bytes=2 * 12 * (c.n + 64))
+        assert st["n_row_only"] > 0
+        want = oracle.component_bc(c.row_ptr, c.col, c.reach)
+        mx, _, zeros = relerr(got, want)
+        assert mx < BC_TOL and zeros
+    finally:
+        g.free()
+
+
+def test_memory_cap_is_an_error():
+    eng = bcx.Engine(0, mem_cap=1 << 20)
+    try:
+        n, uv = bcx.generate(1, 0, 0.5)
+        comps = comps_of(n, uv)
+        with pytest.raises(bcx.BcxError) as e:
+            eng.bc(comps)
+        assert e.value.code == -4
+    finally:
+        eng.close()
+
+
+def test_cli_sample_inputs(goldens, tmp_path):
+    exe = os.path.join(bcx.HERE, "betc")
+    for k in range(1, 5):
+        src = os.path.join(bcx.HERE, "..", "tests", "golden", f"test{k}.txt")
+        out = tmp_path / f"bc{k}.txt"
+        r = subprocess.run([exe, src, "--bc-out-exact", str(out)], capture_output=True, text=True,
+                           cwd=tmp_path, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = r.stdout.splitlines()
+        assert lines[0].startswith(" n_bcc= ")
+        assert lines[-1].startswith("GPU total time is ")
+        assert any(l.startswith("BCC number #: ") for l in lines)
+        bc = np.loadtxt(out)[:, 1]
+        mx, _, zeros = relerr(bc, goldens[f"sample_test{k}"]["bc"])
+        assert mx < BC_TOL and zeros
+        assert (tmp_path / "PostProcessOp.txt").exists()

@@ -1,0 +1,131 @@
+"""Host structure (include/bcx.h part 1) against networkx goldens, brute
+force and invariants; whole-graph parity of the decomposition on the CPU."""
+import numpy as np
+import pytest
+
+import bcx
+import oracle
+from helpers import relerr
+
+
+def test_bcc_and_articulation_vs_networkx(goldens):
+    for name, g in goldens.items():
+        h = bcx.Host.from_edges(g["n"], g["edges"]).decompose()
+        got = sorted(sorted(sorted(map(int, e)) for e in h.bcc_edges(i)) for i in range(h.num_bcc()))
+        assert got == g["bccs"], name
+        assert list(np.nonzero(h.articulation())[0]) == g["articulation"], name
+
+
+def test_tarjan_order_sample1():
+    # Codes/sample_input/test1.txt is one BCC; the first popped edge is the last
+    # back edge pushed by the DFS from vertex 0 in file order.
+    h = bcx.Host.from_file(bcx.os.path.join(bcx.HERE, "..", "tests", "golden", "test1.txt")).decompose()
+    assert h.num_bcc() == 1
+    comps = h.components()
+    assert comps[0].n == 9 and comps[0].arcs == 24
+    # local ids are first-appearance order over the popped edge list
+    uv = h.bcc_edges(0).reshape(-1)
+    seen = []
+    for v in uv:
+        if v not in seen:
+            seen.append(int(v))
+    assert list(comps[0].gid) == seen
+
+
+def test_cut_term_and_reach_vs_bruteforce(goldens):
+    for name, g in goldens.items():
+        n = g["n"]
+        h = bcx.Host.from_edges(n, g["edges"]).decompose()
+        rp, col = oracle.csr_from_edges(n, np.asarray(g["edges"]))
+        np.testing.assert_array_equal(h.cut_term(), oracle.cut_term_brute(rp, col), err_msg=name)
+        for c in h.components():
+            flags = np.zeros(n, np.uint8)
+            flags[c.gid] = 1
+            want = oracle.reach_brute(rp, col, flags)[c.gid]
+            np.testing.assert_array_equal(c.reach, want, err_msg=name)
+
+
+def chains_by_definition(c):
+    deg = np.diff(c.row_ptr)
+    if not (deg >= 3).any():
+        return set()
+    out = set()
+    for L in range(c.n):
+        if deg[L] < 3:
+            continue
+        for j in range(c.row_ptr[L], c.row_ptr[L + 1]):
+            x = int(c.col[j])
+            if deg[x] != 2:
+                continue
+            path, prev, cur = [], L, x
+            while deg[cur] == 2:
+                path.append(cur)
+                a, b = c.col[c.row_ptr[cur]], c.col[c.row_ptr[cur] + 1]
+                prev, cur = cur, int(a if a != prev else b)
+            key = tuple(path) if path[0] < path[-1] or len(path) == 1 and L < cur else tuple(path[::-1])
+            out.add((frozenset([L, cur]), frozenset(path)))
+    return out
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 0.25), (2, 0.003), (4, 0.05), (5, 0.004)])
+def test_chain_invariants(cfg, scale):
+    n, uv = bcx.generate(cfg, 0, scale)
+    h = bcx.Host.from_edges(n, uv).decompose()
+    for ci, c in enumerate(h.components()):
+        got = set()
+        deg = np.diff(c.row_ptr)
+        for k in range(c.n_chains):
+            nodes = c.chain_nodes[c.chain_ptr[k]:c.chain_ptr[k + 1]]
+            L, R = int(c.chain_ends[2 * k]), int(c.chain_ends[2 * k + 1])
+            assert L != R and deg[L] >= 3 and deg[R] >= 3
+            assert (deg[nodes] == 2).all()
+            # consecutive along the path, ld = j+1 / rd = cnt-j distances
+            seq = [L] + list(map(int, nodes)) + [R]
+            for a, b in zip(seq, seq[1:]):
+                assert b in c.col[c.row_ptr[a]:c.row_ptr[a + 1]]
+            got.add((frozenset([L, R]), frozenset(map(int, nodes))))
+        assert got == chains_by_definition(c)
+        f, e, x = h.component_counts(ci)
+        assert f + e + x == c.n
+        assert x == len(c.chain_nodes)
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 0.25), (2, 0.002), (4, 0.02), (5, 0.002)])
+def test_pipeline_equals_whole_graph_brandes(cfg, scale):
+    """cut term + weighted component BC == plain Brandes on the input graph."""
+    n, uv = bcx.generate(cfg, 0, scale)
+    h = bcx.Host.from_edges(n, uv).decompose()
+    rp, col = oracle.csr_from_edges(n, uv)
+    whole = oracle.brandes_bc(rp, col, threads=4)
+    piped = oracle.pipeline_bc(h, threads=4)
+    mx, mean, zeros = relerr(piped, whole)
+    assert mx < 1e-12 and zeros
+
+
+def test_input_checks():
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.from_edges(3, [(0, 1), (1, 1)]).decompose()
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.from_edges(3, [(0, 1), (1, 0), (1, 2)]).decompose()
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.from_edges(4, [(0, 1), (2, 3)]).decompose()
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.from_file("/nonexistent/graph.txt")
+
+
+def test_file_reader_matches_edges(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("4 4\n0 1 1\n1 2 1\n2 3 1\n3 0 1\n")
+    h = bcx.Host.from_file(str(p)).decompose()
+    assert h.n == 4 and h.m == 4 and h.num_bcc() == 1
+    p.write_text("4 2\n0 1 1\n1 2 2\n")
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.from_file(str(p))
+
+
+def test_generators_deterministic_and_simple():
+    for cfg in (1, 2, 3, 4, 5):
+        n1, uv1 = bcx.generate(cfg, 0, 0.002 if cfg != 1 else 0.2)
+        n2, uv2 = bcx.generate(cfg, 0, 0.002 if cfg != 1 else 0.2)
+        assert n1 == n2 and np.array_equal(uv1, uv2)
+        bcx.Host.from_edges(n1, uv1).decompose(check_input=True)  # simple + connected
