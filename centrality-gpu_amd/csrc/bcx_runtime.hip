@@ -663,7 +663,6 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
         Rows rows{};
         if ((rc = ensure_rows(ctx, n, 2, rows))) break;
         PlainScratch sc;
-        if ((rc = ensure_plain_scratch(ctx, n, 1, sc))) break;
         double *bc = (double *)ctx->bc_acc.p;
         if (ci < 0) {
             std::vector<int32_t> s{src}, sl{0};
@@ -671,6 +670,7 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
             rc = run_bfs_list(ctx, dc, s, sl, m, f64, limit, bc, rows, nullptr);
             if (rc) break;
             if (ctx->overflow && !f64) { f64 = true; continue; }
+            if ((rc = ensure_plain_scratch(ctx, n, 1, sc))) break;
             rc = ctx_hip(ctx, hipMemcpy(lv.data(), rows.d, 4 * n, hipMemcpyDeviceToHost), "copy");
             if (!rc) rc = ctx_hip(ctx, hipMemcpy(sg.data(), rows.sig, 8 * n, hipMemcpyDeviceToHost), "copy");
         } else {
@@ -685,6 +685,8 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
             b.cslots = {0, 1};
             rc = run_chain_list(ctx, dc, b, f64, bc, rows, nullptr);
             if (rc) break;
+            // the chain launch ran one workgroup: its rows start at offset 0
+            if ((rc = ensure_plain_scratch(ctx, n, 1, sc))) break;
             rc = ctx_hip(ctx, hipMemcpy(lv.data(), sc.d, 4 * n, hipMemcpyDeviceToHost), "copy");
             if (!rc) rc = ctx_hip(ctx, hipMemcpy(sg.data(), sc.sig, 8 * n, hipMemcpyDeviceToHost), "copy");
         }
