@@ -1,0 +1,39 @@
+// Device view and launchers of the batched multi-source kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <type_traits>
+
+namespace bcx {
+
+constexpr int BATCH_W = 64;  // sources per tile (one per wave64 lane)
+
+struct BatchDev {
+    int32_t n;
+    const int32_t *rp, *col, *reach;
+    uint64_t *vis, *mb0, *mb1, *mb2;  // lanes reached; rotating level masks
+    uint64_t *sig;                    // [n*64] u64 or f64 bits
+    void *tt;                         // [n*64] float or double
+    uint16_t *lvl;                    // optional [n*64] levels (probe)
+    double *dbg_delta;                // optional [n*64] dependencies (probe)
+    int32_t *Q;                       // level lists, concatenated
+    uint64_t *Qm;                     // lane mask of each list entry
+    int32_t *ctr;                     // [0] list end, [1] sigma overflow, [2] list overflow
+    int32_t qcap;
+    const int32_t *srcs;              // [64] source per lane
+    const double *ws;                 // [64] source weight r(s)+1 per lane
+    double *bc;                       // component slice of the BC accumulator
+    uint64_t limit;                   // u64 sigma threshold (0 = 2^64)
+};
+
+hipError_t launch_b_init(const BatchDev &b, int ns, bool f64, int cap, hipStream_t st);
+hipError_t launch_b_expand(const BatchDev &b, int l, int qm0, int q0, int q1, int cap, hipStream_t st);
+hipError_t launch_b_pull(const BatchDev &b, bool f64, int l, int q1, int q2, int cap, hipStream_t st);
+hipError_t launch_b_back(const BatchDev &b, bool f64, bool fp32, int l, int q0, int q1, int c0, int c1, int cap,
+                         hipStream_t st);
+hipError_t launch_b_clear2(const BatchDev &b, int ai, int a0, int a1, int bj, int b0, int b1, int cap,
+                           hipStream_t st);
+
+}  // namespace bcx

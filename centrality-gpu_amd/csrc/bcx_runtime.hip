@@ -596,7 +596,9 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
     s->t_kernel_s = ctx->timer.collect() + s->t_chain_s;
     if (bc_out && g->total_n > 0) {
         std::vector<double> h((size_t)g->total_n);
-        rc = ctx_hip(ctx, hipMemcpy(h.data(), bc, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "bc copy");
+        rc = ctx_hip(ctx, hipMemcpyAsync(h.data(), bc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream),
+                     "bc copy");
+        if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "bc copy");
         if (rc) return rc;
         for (size_t i = 0; i < h.size(); ++i) bc_out[i] += h[i];
     }
@@ -671,8 +673,8 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
             if (rc) break;
             if (ctx->overflow && !f64) { f64 = true; continue; }
             if ((rc = ensure_plain_scratch(ctx, n, 1, sc))) break;
-            rc = ctx_hip(ctx, hipMemcpy(lv.data(), rows.d, 4 * n, hipMemcpyDeviceToHost), "copy");
-            if (!rc) rc = ctx_hip(ctx, hipMemcpy(sg.data(), rows.sig, 8 * n, hipMemcpyDeviceToHost), "copy");
+            rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), rows.d, 4 * n, hipMemcpyDeviceToHost, ctx->stream), "copy");
+            if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), rows.sig, 8 * n, hipMemcpyDeviceToHost, ctx->stream), "copy");
         } else {
             std::vector<int32_t> s{dc.host.ch_ends[2 * ci], dc.host.ch_ends[2 * ci + 1]}, sl{0, 1};
             std::vector<int8_t> m{MODE_ROW, MODE_ROW};
@@ -687,10 +689,11 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
             if (rc) break;
             // the chain launch ran one workgroup: its rows start at offset 0
             if ((rc = ensure_plain_scratch(ctx, n, 1, sc))) break;
-            rc = ctx_hip(ctx, hipMemcpy(lv.data(), sc.d, 4 * n, hipMemcpyDeviceToHost), "copy");
-            if (!rc) rc = ctx_hip(ctx, hipMemcpy(sg.data(), sc.sig, 8 * n, hipMemcpyDeviceToHost), "copy");
+            rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), sc.d, 4 * n, hipMemcpyDeviceToHost, ctx->stream), "copy");
+            if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), sc.sig, 8 * n, hipMemcpyDeviceToHost, ctx->stream), "copy");
         }
-        if (!rc) rc = ctx_hip(ctx, hipMemcpy(dl.data(), sc.dl, 8 * n, hipMemcpyDeviceToHost), "copy");
+        if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), sc.dl, 8 * n, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "copy");
         break;
     }
     (void)ctx->timer.collect();

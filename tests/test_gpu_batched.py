@@ -1,0 +1,114 @@
+"""GPU parity of the batched multi-source path through the C-ABI.
+
+Bar: levels and sigma bit-exact against the oracle (uint64, and FP64 sigma in
+the oracle's CSR pull order); with FP64 dependency terms delta within 1e-12
+and BC within 1e-9 relative; with FP32-compensated terms delta within 2e-6
+and BC within 1e-6 max relative; zero BC exact.
+"""
+import numpy as np
+import pytest
+
+import bcx
+import oracle
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+B = bcx.PATH_BATCHED
+
+
+def comps_of(n, uv, chains=True):
+    return bcx.Host.from_edges(n, uv).decompose(use_chains=chains).components()
+
+
+def check_source(engine, c, s, prec, **kw):
+    lv, su, sf, dl, isf = engine.debug_source(c, s, path=B, precision=prec, **kw)
+    olv, osu, osf, odl, ovf = oracle.source(c.row_ptr, c.col, c.reach, s)
+    np.testing.assert_array_equal(lv, olv)
+    if isf:
+        np.testing.assert_array_equal(sf, osf)
+    else:
+        assert not ovf
+        np.testing.assert_array_equal(su, osu)
+    tol = 2e-6 if prec == bcx.PREC_FP32C else 1e-12
+    np.testing.assert_allclose(dl, odl, rtol=tol, atol=1e-300)
+
+
+@pytest.mark.parametrize("prec", [bcx.PREC_FP64, bcx.PREC_FP32C])
+def test_levels_sigma_delta_goldens(engine, goldens, prec):
+    for name, g in goldens.items():
+        print("case", name, prec, flush=True)
+        for c in comps_of(g["n"], g["edges"], chains=False):
+            for s in range(0, c.n, max(1, c.n // 7)):
+                check_source(engine, c, s, prec)
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 0.25), (2, 0.003), (4, 0.05), (5, 0.003)])
+def test_sampled_sources_generated(engine, cfg, scale):
+    n, uv = bcx.generate(cfg, 0, scale)
+    big = max(comps_of(n, uv), key=lambda c: c.n)
+    rng = np.random.default_rng(cfg)
+    print("case cfg", cfg, "n", big.n, flush=True)
+    for s in rng.choice(big.n, size=min(8, big.n), replace=False):
+        check_source(engine, big, int(s), bcx.PREC_FP32C)
+
+
+@pytest.mark.parametrize("prec,tol", [(bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)])
+def test_component_bc_goldens(engine, goldens, prec, tol):
+    for name, g in goldens.items():
+        comps = comps_of(g["n"], g["edges"])
+        if not comps:
+            continue
+        got, st = engine.bc(comps, path=B, precision=prec)
+        for c, b in zip(comps, got):
+            mx, _, zeros = relerr(b, oracle.component_bc(c.row_ptr, c.col, c.reach))
+            assert mx < tol and zeros, name
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 1.0), (2, 0.003), (4, 0.1), (5, 0.003)])
+@pytest.mark.parametrize("prec,tol", [(bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)])
+def test_whole_graph_bc_generated(engine, cfg, scale, prec, tol):
+    n, uv = bcx.generate(cfg, 0, scale)
+    h = bcx.Host.from_edges(n, uv).decompose()
+    rp, col = oracle.csr_from_edges(n, uv)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    print("case cfg", cfg, "n", n, "prec", prec, flush=True)
+    bc, st = engine.host_bc(h, path=B, precision=prec)
+    mx, mean, zeros = relerr(bc, want)
+    print("maxrel", mx, "mean", mean, flush=True)
+    assert mx < tol and zeros
+
+
+def test_fp64_sigma_grid(engine):
+    w = 40
+    edges = []
+    for y in range(w):
+        for x in range(w):
+            v = y * w + x
+            if x + 1 < w:
+                edges.append((v, v + 1))
+            if y + 1 < w:
+                edges.append((v, v + w))
+    c = comps_of(w * w, edges)[0]
+    for s in (0, w - 1, w * w // 2 + 3):
+        check_source(engine, c, s, bcx.PREC_FP64)
+    got, st = engine.bc([c], path=B, precision=bcx.PREC_FP64)
+    assert st["n_f64_sigma"] > 0
+    mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
+    assert mx < 1e-9 and zeros
+
+
+def test_threshold_hook_and_subset(engine):
+    n, uv = bcx.generate(4, 0, 0.05)
+    c = max(comps_of(n, uv), key=lambda c: c.n)
+    got, st = engine.bc([c], path=B, precision=bcx.PREC_FP32C, sigma_threshold_log2=6)
+    assert st["n_f64_sigma"] > 0
+    mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
+    assert mx < 1e-6 and zeros
+    g = engine.upload([c])
+    try:
+        sub = np.sort(np.random.default_rng(5).choice(c.n, size=c.n // 3, replace=False))
+        got, st = g.run(path=B, precision=bcx.PREC_FP32C, subset=sub)
+        mx, _, zeros = relerr(got, oracle.component_bc(c.row_ptr, c.col, c.reach, sources=sub))
+        assert mx < 1e-6 and zeros
+    finally:
+        g.free()
