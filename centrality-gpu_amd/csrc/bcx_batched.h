@@ -20,7 +20,9 @@ struct BatchDev {
     double *dbg_delta;                // optional [n*64] dependencies (probe)
     int32_t *Q;                       // level lists, concatenated
     uint64_t *Qm;                     // lane mask of each list entry
-    int32_t *ctr;                     // [0] list end, [1] sigma overflow, [2] list overflow
+    int32_t *ctr;                     // [0] list end, [1] sigma overflow, [2] list overflow,
+                                      // [3] depth (-1 while the forward runs)
+    int32_t *qoff;                    // level offsets into Q (device-maintained)
     int32_t qcap;
     const int32_t *srcs;              // [64] source per lane
     const double *ws;                 // [64] source weight r(s)+1 per lane
@@ -28,12 +30,12 @@ struct BatchDev {
     uint64_t limit;                   // u64 sigma threshold (0 = 2^64)
 };
 
+// Every level kernel reads its list ranges from b.qoff / b.ctr, so a whole
+// tile is a fixed launch sequence with no host round trip per level.
 hipError_t launch_b_init(const BatchDev &b, int ns, bool f64, int cap, hipStream_t st);
-hipError_t launch_b_expand(const BatchDev &b, int l, int qm0, int q0, int q1, int cap, hipStream_t st);
-hipError_t launch_b_pull(const BatchDev &b, bool f64, int l, int q1, int q2, int cap, hipStream_t st);
-hipError_t launch_b_back(const BatchDev &b, bool f64, bool fp32, int l, int q0, int q1, int c0, int c1, int cap,
-                         hipStream_t st);
-hipError_t launch_b_clear2(const BatchDev &b, int ai, int a0, int a1, int bj, int b0, int b1, int cap,
-                           hipStream_t st);
+hipError_t launch_b_forward(const BatchDev &b, bool f64, int l, int grid, hipStream_t st);
+hipError_t launch_b_back(const BatchDev &b, bool f64, bool fp32, int l, int grid, hipStream_t st);
+hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st);
+hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st);
 
 }  // namespace bcx

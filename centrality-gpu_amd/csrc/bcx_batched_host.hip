@@ -14,22 +14,32 @@
 using namespace bcx;
 
 struct BatchedState {
-    DBuf vis, mb, sig, tt, Q, Qm, ctr, srcs, ws, lvl, dbg;
+    DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, srcs, ws, lvl, dbg;
     int32_t n_cap = 0;
     int32_t qcap = 0;
+    int32_t src_cap = 0;
     bool probe = false;
+    int32_t *pinned = nullptr;  // ctr read-back
+    int last_depth = 8;
 };
 
-static int bs_ensure(bcx_ctx *ctx, int32_t n, bool probe) {
+static int bs_ensure(bcx_ctx *ctx, int32_t n, int32_t n_src, bool probe) {
     if (!ctx->batched) ctx->batched = new BatchedState();
     BatchedState &s = *ctx->batched;
     int rc;
     const size_t N = (size_t)n;
+    if (!s.pinned) {
+        if (hipHostMalloc((void **)&s.pinned, 64, hipHostMallocDefault) != hipSuccess) {
+            s.pinned = nullptr;
+            return ctx_fail(ctx, BCX_E_HIP, "pinned counters");
+        }
+    }
     if (n > s.n_cap) {
         if ((rc = s.vis.ensure(ctx, 8 * N, "batched vis")) ||
             (rc = s.mb.ensure(ctx, 24 * N, "batched masks")) ||
             (rc = s.sig.ensure(ctx, 8 * 64 * N, "batched sigma")) ||
-            (rc = s.tt.ensure(ctx, 8 * 64 * N, "batched dependency terms")))
+            (rc = s.tt.ensure(ctx, 8 * 64 * N, "batched dependency terms")) ||
+            (rc = s.qoff.ensure(ctx, 4 * (N + 8), "batched level offsets")))
             return rc;
         rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, ctx->stream), "memset");
         if (rc) return rc;
@@ -42,9 +52,13 @@ static int bs_ensure(bcx_ctx *ctx, int32_t n, bool probe) {
             return rc;
         s.qcap = (int32_t)q;
     }
-    if ((rc = s.ctr.ensure(ctx, 64, "batched counters")) || (rc = s.srcs.ensure(ctx, 4 * 64, "batched sources")) ||
-        (rc = s.ws.ensure(ctx, 8 * 64, "batched weights")))
-        return rc;
+    if (n_src > s.src_cap) {
+        if ((rc = s.srcs.ensure(ctx, 4 * (size_t)n_src, "batched sources")) ||
+            (rc = s.ws.ensure(ctx, 8 * (size_t)n_src, "batched weights")))
+            return rc;
+        s.src_cap = n_src;
+    }
+    if ((rc = s.ctr.ensure(ctx, 64, "batched counters"))) return rc;
     if (probe) {
         if ((rc = s.lvl.ensure(ctx, 2 * 64 * N, "probe levels")) || (rc = s.dbg.ensure(ctx, 8 * 64 * N, "probe delta")))
             return rc;
@@ -53,7 +67,7 @@ static int bs_ensure(bcx_ctx *ctx, int32_t n, bool probe) {
     return BCX_OK;
 }
 
-static BatchDev make_dev(bcx_ctx *ctx, const DComp &dc, double *bc, uint64_t limit) {
+static BatchDev make_dev(bcx_ctx *ctx, const DComp &dc, double *bc, uint64_t limit, int tile) {
     BatchedState &s = *ctx->batched;
     BatchDev b{};
     b.n = dc.n;
@@ -71,101 +85,94 @@ static BatchDev make_dev(bcx_ctx *ctx, const DComp &dc, double *bc, uint64_t lim
     b.Q = (int32_t *)s.Q.p;
     b.Qm = (uint64_t *)s.Qm.p;
     b.ctr = (int32_t *)s.ctr.p;
+    b.qoff = (int32_t *)s.qoff.p;
     b.qcap = s.qcap;
-    b.srcs = (const int32_t *)s.srcs.p;
-    b.ws = (const double *)s.ws.p;
+    b.srcs = (const int32_t *)s.srcs.p + (size_t)tile * BATCH_W;
+    b.ws = (const double *)s.ws.p + (size_t)tile * BATCH_W;
     b.bc = bc;
     b.limit = limit;
     return b;
 }
 
-static int read_ctr(bcx_ctx *ctx, int32_t *h) {
-    int rc = ctx_hip(ctx, hipMemcpyAsync(h, ctx->batched->ctr.p, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                         ctx->stream), "batched counters");
-    if (rc) return rc;
-    return ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "batched kernels");
-}
-
-// One tile; on return *qoff_out holds the level offsets of the lists.
-static int run_tile(bcx_ctx *ctx, const DComp &dc, const std::vector<int32_t> &src, bool fp32, double *bc,
-                    uint64_t limit, bcx_stats *st, std::vector<int32_t> *qoff_out, bool *f64_out) {
+// One tile: init, forward levels and a speculative backward launched as one
+// sequence, then a single read-back of the counters.  A forward deeper than
+// the launched levels continues from where it stopped; a sigma overflow
+// (uint64) or full level lists leave the BC untouched and redo the tile.
+static int run_tile(bcx_ctx *ctx, const DComp &dc, int tile, int ns, bool fp32, double *bc, uint64_t limit,
+                    bcx_stats *st, bool *f64_out) {
     BatchedState &s = *ctx->batched;
-    const int ns = (int)src.size();
-    int32_t hs[64];
-    double hw[64];
-    for (int k = 0; k < 64; ++k) {
-        hs[k] = k < ns ? src[k] : -1;
-        hw[k] = k < ns ? (double)dc.host.reach[src[k]] + 1.0 : 0.0;
-    }
-    int rc;
-    if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.srcs.p, hs, sizeof(hs), hipMemcpyHostToDevice, ctx->stream), "srcs")) ||
-        (rc = ctx_hip(ctx, hipMemcpyAsync(s.ws.p, hw, sizeof(hw), hipMemcpyHostToDevice, ctx->stream), "ws")))
-        return rc;
-    const int cap = ctx->n_cu * 8;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)dc.n + 3) / 4, (int64_t)ctx->n_cu * 8));
     bool f64 = false;
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        BatchDev b = make_dev(ctx, dc, bc, f64 ? 0 : limit);
-        std::vector<int32_t> qoff{0, ns};
+    int rc;
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        BatchDev b = make_dev(ctx, dc, bc, f64 ? 0 : limit, tile);
         ctx->timer.begin(ctx);
-        if ((rc = ctx_hip(ctx, launch_b_init(b, ns, f64, cap, ctx->stream), "init"))) return rc;
-        int l = 0, D = 0;
-        int32_t h[3];
-        bool qfull = false;
+        if ((rc = ctx_hip(ctx, launch_b_init(b, ns, f64, grid, ctx->stream), "init"))) return rc;
+        int L = 0, C = std::max(4, s.last_depth + 3);
+        int32_t *h = s.pinned;
+        int launches = 2;
         while (true) {
-            const int qm0 = l > 0 ? qoff[l - 1] : 0;
-            if ((rc = ctx_hip(ctx, launch_b_expand(b, l, qm0, qoff[l], qoff[l + 1], cap, ctx->stream), "expand")))
+            for (int l = L; l < L + C; ++l)
+                if ((rc = ctx_hip(ctx, launch_b_forward(b, f64, l, grid, ctx->stream), "forward"))) return rc;
+            L += C;
+            for (int l = L - 1; l >= 1; --l)
+                if ((rc = ctx_hip(ctx, launch_b_back(b, f64, fp32, l, grid, ctx->stream), "back"))) return rc;
+            if ((rc = ctx_hip(ctx, launch_b_finish(b, grid, ctx->stream), "finish"))) return rc;
+            launches += 2 * C + (L - 1) + 1;
+            if ((rc = ctx_hip(ctx, hipMemcpyAsync(h, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+                              "counters")) ||
+                (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "batched kernels")))
                 return rc;
-            if ((rc = read_ctr(ctx, h))) return rc;
-            if (h[2]) { qfull = true; break; }
-            qoff.push_back(h[0]);
-            if (h[0] == qoff[l + 1]) { D = l; break; }
-            if ((rc = ctx_hip(ctx, launch_b_pull(b, f64, l, qoff[l + 1], h[0], cap, ctx->stream), "pull")))
-                return rc;
-            l++;
+            if (h[3] >= 0) break;
+            C = std::min(2 * C, 4096);
         }
-        if (!qfull && !f64) {
-            if ((rc = read_ctr(ctx, h))) return rc;
-        }
-        if (qfull || (!f64 && h[1])) {
-            // clean every mask entry the aborted forward may have set, then retry
-            const int end = std::min<int32_t>(qfull ? s.qcap : h[0], s.qcap);
-            if ((rc = ctx_hip(ctx, launch_b_clear2(b, 0, 0, end, 1, 0, end, cap, ctx->stream), "clear")) ||
-                (rc = ctx_hip(ctx, launch_b_clear2(b, 2, 0, end, 2, 0, 0, cap, ctx->stream), "clear")))
+        ctx->timer.end(ctx);
+        if (st) st->n_launches += launches;
+        const int D = h[3];
+        if (h[2]) {
+            // lists full: masks may hold entries of vertices never listed
+            if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, ctx->stream), "memset"))) return rc;
+            const int64_t q = std::min<int64_t>((int64_t)s.qcap * 4, std::min<int64_t>((int64_t)64 * dc.n + 4096,
+                                                                                        INT32_MAX / 2));
+            if (q <= s.qcap) return ctx_fail(ctx, BCX_E_NOMEM, "batched level lists cannot grow");
+            if ((rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "sync")) ||
+                (rc = s.Q.ensure(ctx, 4 * (size_t)q, "batched lists")) ||
+                (rc = s.Qm.ensure(ctx, 8 * (size_t)q, "batched lists")))
                 return rc;
-            ctx->timer.end(ctx);
-            if (qfull) {
-                s.qcap = 0;
-                const int64_t q = std::min<int64_t>((int64_t)64 * dc.n + 4096, INT32_MAX / 2);
-                (void)hipStreamSynchronize(ctx->stream);
-                if ((rc = s.Q.ensure(ctx, 4 * (size_t)q, "batched lists")) ||
-                    (rc = s.Qm.ensure(ctx, 8 * (size_t)q, "batched lists")))
-                    return rc;
-                s.qcap = (int32_t)q;
-            } else {
-                f64 = true;
-                if (st) st->n_f64_sigma += ns;
-            }
+            s.qcap = (int32_t)q;
             continue;
         }
-        if (st && D > st->max_level) st->max_level = D;
-        // backward, levels D .. 1
-        for (int lb = D; lb >= 1; --lb) {
-            const int c0 = lb + 2 <= D ? qoff[lb + 2] : 0, c1 = lb + 2 <= D ? qoff[lb + 3] : 0;
-            if ((rc = ctx_hip(ctx, launch_b_back(b, f64, fp32, lb, qoff[lb], qoff[lb + 1], c0, c1, cap, ctx->stream),
-                              "back")))
+        if (!f64 && h[1]) {
+            if ((rc = ctx_hip(ctx, launch_b_clear_all(b, std::min(h[0], s.qcap), grid, ctx->stream), "clear")))
                 return rc;
+            f64 = true;
+            if (st) st->n_f64_sigma += ns;
+            continue;
         }
-        // masks of levels 1 and 2 remain from the backward
-        const int a0 = qoff[1], a1 = D >= 1 ? qoff[2] : qoff[1];
-        const int b0 = D >= 2 ? qoff[2] : 0, b1 = D >= 2 ? qoff[3] : 0;
-        if ((rc = ctx_hip(ctx, launch_b_clear2(b, 1, a0, a1, 2, b0, b1, cap, ctx->stream), "clear"))) return rc;
-        ctx->timer.end(ctx);
-        if (st) st->n_launches += 2 + 2 * D + D + 1;
-        if (qoff_out) *qoff_out = qoff;
+        s.last_depth = D;
+        if (st && D > st->max_level) st->max_level = D;
         if (f64_out) *f64_out = f64;
         return BCX_OK;
     }
     return ctx_fail(ctx, BCX_E_OVERFLOW, "batched tile could not be completed");
+}
+
+static int stage_sources(bcx_ctx *ctx, const DComp &dc, const std::vector<int32_t> &srcs) {
+    const size_t ntiles = (srcs.size() + BATCH_W - 1) / BATCH_W;
+    std::vector<int32_t> hs(ntiles * BATCH_W, -1);
+    std::vector<double> hw(ntiles * BATCH_W, 0.0);
+    for (size_t i = 0; i < srcs.size(); ++i) {
+        hs[i] = srcs[i];
+        hw[i] = (double)dc.host.reach[srcs[i]] + 1.0;
+    }
+    BatchedState &s = *ctx->batched;
+    int rc;
+    if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.srcs.p, hs.data(), 4 * hs.size(), hipMemcpyHostToDevice, ctx->stream),
+                      "sources")) ||
+        (rc = ctx_hip(ctx, hipMemcpyAsync(s.ws.p, hw.data(), 8 * hw.size(), hipMemcpyHostToDevice, ctx->stream),
+                      "weights")))
+        return rc;
+    return ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "sources");
 }
 
 // BFS order from the highest-degree vertex (host).
@@ -191,10 +198,6 @@ static std::vector<int32_t> bfs_rank(const HostComp &hc) {
 
 int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, const bcx_opts *o, double *bc,
                       bcx_stats *st) {
-    int rc = bs_ensure(ctx, dc.n, false);
-    if (rc) return rc;
-    const uint64_t limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
-                               ? (1ull << o->sigma_f64_threshold_log2) : 0;
     // every requested source runs as a BFS lane: the plan's BFS sources plus
     // its chain sources
     std::vector<int32_t> srcs(plan.free_src);
@@ -207,11 +210,16 @@ int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, c
     const std::vector<int32_t> rank = bfs_rank(dc.host);
     std::sort(srcs.begin(), srcs.end(), [&](int32_t a, int32_t b) { return rank[a] < rank[b]; });
     srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
+    const size_t ntiles = (srcs.size() + BATCH_W - 1) / BATCH_W;
+    int rc = bs_ensure(ctx, dc.n, (int32_t)(ntiles * BATCH_W), false);
+    if (rc) return rc;
+    if ((rc = stage_sources(ctx, dc, srcs))) return rc;
+    const uint64_t limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
+                               ? (1ull << o->sigma_f64_threshold_log2) : 0;
     const bool fp32 = o->precision == BCX_PREC_FP32C;
-    std::vector<int32_t> tile;
-    for (size_t i = 0; i < srcs.size(); i += BATCH_W) {
-        tile.assign(srcs.begin() + i, srcs.begin() + std::min(srcs.size(), i + BATCH_W));
-        if ((rc = run_tile(ctx, dc, tile, fp32, bc, limit, st, nullptr, nullptr))) return rc;
+    for (size_t t = 0; t < ntiles; ++t) {
+        const int ns = (int)std::min<size_t>(BATCH_W, srcs.size() - t * BATCH_W);
+        if ((rc = run_tile(ctx, dc, (int)t, ns, fp32, bc, limit, st, nullptr))) return rc;
     }
     return BCX_OK;
 }
@@ -223,7 +231,7 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
     if (rc) return rc;
     DComp &dc = g->comps[0];
     const int32_t n = dc.n;
-    rc = bs_ensure(ctx, n, true);
+    rc = bs_ensure(ctx, n, BATCH_W, true);
     if (!rc) rc = ctx->bc_acc.ensure(ctx, sizeof(double) * n, "bc accumulator");
     bool f64 = false;
     std::vector<uint16_t> lv((size_t)n * 64);
@@ -233,8 +241,10 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
         const uint64_t limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
                                    ? (1ull << o->sigma_f64_threshold_log2) : 0;
         std::vector<int32_t> tile{src};
-        rc = run_tile(ctx, dc, tile, o->precision == BCX_PREC_FP32C, (double *)ctx->bc_acc.p, limit, nullptr, nullptr,
-                      &f64);
+        rc = stage_sources(ctx, dc, tile);
+        if (!rc)
+            rc = run_tile(ctx, dc, 0, 1, o->precision == BCX_PREC_FP32C, (double *)ctx->bc_acc.p, limit, nullptr,
+                          &f64);
     }
     // copies are ordered on the context stream (the kernels run there)
     if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), ctx->batched->lvl.p, 2 * lv.size(), hipMemcpyDeviceToHost,
@@ -264,7 +274,9 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
 void batched_release(bcx_ctx *ctx) {
     if (!ctx->batched) return;
     BatchedState &s = *ctx->batched;
-    for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.srcs, &s.ws, &s.lvl, &s.dbg}) b->release(ctx);
+    for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.qoff, &s.srcs, &s.ws, &s.lvl, &s.dbg})
+        b->release(ctx);
+    if (s.pinned) (void)hipHostFree(s.pinned);
     delete ctx->batched;
     ctx->batched = nullptr;
 }
