@@ -24,6 +24,16 @@ struct BatchDev {
                                       // [3] depth (-1 while the forward runs)
     int32_t *qoff;                    // level offsets into Q (device-maintained)
     int32_t qcap;
+    // hub vertices (degree > hub_deg): their arcs are cut into static slices
+    // so several waves share one hub; n_hub == 0 disables the split
+    int32_t n_hub, n_slice, hub_deg, hub_stride;
+    const int32_t *hub_v;             // [n_hub] vertex of each hub
+    const int32_t *hub_nsl;           // [n_hub] slices per hub
+    const int32_t *sl_hub;            // [n_slice] hub of each slice
+    const int32_t *sl_j0, *sl_j1;     // [n_slice] arc range of each slice
+    uint64_t *hub_mask;               // [n_hub * hub_stride] lanes of hub at level l
+    double *hub_acc;                  // [n_hub * 64] backward partial sums
+    int32_t *hub_cnt;                 // [n_hub] slices finished this level
     const int32_t *srcs;              // [64] source per lane
     const double *ws;                 // [64] source weight r(s)+1 per lane
     double *bc;                       // component slice of the BC accumulator

@@ -10,20 +10,27 @@
 //   Q / Qm        per-level lists of (vertex, lane mask), levels concatenated,
 //                 level l = Q[qoff[l], qoff[l+1])
 //
-// Forward level l (two launches):
-// expand pushes lane masks from Q_l to neighbours with one
-// 64-bit atomic OR per (arc, new lanes) and appends newly touched vertices
-// (wave-aggregated) to Q_{l+1}; pull then computes, for each w in Q_{l+1},
-// sigma_w = sum of sigma_v over neighbours v in CSR order for the lanes where
-// v is at level l -- no sigma atomics, and the FP64 order is the oracle's, so
-// FP64 sigma is bit-exact (Codes/kernels_free.cu:286-362 computes the same
-// quantities one source per block with atomics).
+// Forward level l, two launches:
+//   expand  pushes lane masks from Q_l to neighbours, F & ~vis[w], with
+//           fire-and-forget 64-bit atomic ORs into mb[(l+1)%3] (a lane walks
+//           the arcs of a low-degree entry, a wave those of a larger one);
+//   pull    scans mb[(l+1)%3] in coalesced 64-vertex chunks; every non-zero
+//           entry w joins Q_{l+1} and gets sigma_w = sum of sigma_v over
+//           neighbours v in CSR order for the lanes where v is at level l --
+//           no sigma atomics, and the FP64 order is the oracle's, so FP64
+//           sigma is bit-exact.  (Codes/kernels_free.cu:286-362 computes the
+//           same quantities one source per block with atomics.)
 // Backward level l: rho_v = sum over successors w of t_w, with
 // rho = delta / sigma, so the successor-method term
 // sigma_v/sigma_w (r(w)+1 + delta_w) (Codes/kernels_free.cu:378-421) becomes a
 // plain sum of one gathered word per (arc, lane); FP32 with Kahan
 // compensation (or FP64).  BC(v) += sum_lanes (r(s)+1) sigma_v rho_v
 // (Codes/kernels_free.cu:423-426), one FP64 atomic per (vertex, level).
+//
+// Hubs (degree > hub_deg) are cut into static arc slices shared by several
+// waves: uint64 sigma partials are added with 64-bit atomics (exact), rho
+// partials into FP64 accumulators and the last slice to finish a hub
+// finalises it.  FP64-sigma tiles keep hubs whole in the pull (fixed order).
 //
 // Every kernel reads its list ranges from the device (qoff, ctr) and returns
 // at once when the forward has finished (ctr[3] >= 0), so the host launches a
@@ -33,8 +40,11 @@
 
 namespace bcx {
 
-constexpr int BB = 256;  // threads per block of the level kernels
-constexpr int UNR = 8;   // gathers in flight per lane in the sum loops
+constexpr int BB = 256;        // threads per block of the level kernels
+constexpr int NW = BB / 64;    // waves per block
+constexpr int UNR = 8;         // gathers in flight per lane in the sum loops
+constexpr int SMALL = 16;      // degree up to which one lane expands an entry
+constexpr int STAGE = 256, STAGE_FLUSH = 192;
 
 __device__ __forceinline__ int gwave() { return (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6); }
 __device__ __forceinline__ int nwaves() { return (int)((gridDim.x * (unsigned)blockDim.x) >> 6); }
@@ -48,9 +58,24 @@ __device__ __forceinline__ uint64_t *mbuf(const BatchDev &b, int l) {
     return r == 0 ? b.mb0 : (r == 1 ? b.mb1 : b.mb2);
 }
 
+__device__ __forceinline__ bool is_hub(const BatchDev &b, int deg) { return b.n_hub > 0 && deg > b.hub_deg; }
+
+__device__ __forceinline__ int ldi(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------- init ---
 __global__ __launch_bounds__(BB) void k_b_clear_vis(BatchDev b) {
-    for (int64_t v = blockIdx.x * (int64_t)BB + threadIdx.x; v < b.n; v += (int64_t)gridDim.x * BB) b.vis[v] = 0;
+    const int64_t stride = (int64_t)gridDim.x * BB;
+    const int64_t t0 = blockIdx.x * (int64_t)BB + threadIdx.x;
+    for (int64_t v = t0; v < b.n; v += stride) b.vis[v] = 0;
+    // hub sigma rows and backward accumulators start at zero every tile: each
+    // (hub, lane) is summed into exactly once
+    for (int64_t i = t0; i < (int64_t)b.n_hub * 64; i += stride) {
+        b.sig[(int64_t)b.hub_v[i >> 6] * 64 + (i & 63)] = 0;
+        b.hub_acc[i] = 0.0;
+    }
+    for (int64_t i = t0; i < b.n_hub; i += stride) b.hub_cnt[i] = 0;
 }
 
 template <bool F64>
@@ -72,137 +97,197 @@ __global__ void k_b_seed(BatchDev b, int ns) {
         b.ctr[2] = 0;
         b.ctr[3] = -1;
         b.qoff[0] = 0;
-        b.qoff[1] = ns;
     }
 }
 
 // ------------------------------------------------------------- expand ---
-// Q_l -> candidates of Q_{l+1}; clears Q_{l-1} from mask array (l+2)%3.
-__device__ void expand_level(const BatchDev &b, int l) {
-    const int q0 = b.qoff[l], q1 = b.qoff[l + 1];
-    const int qm0 = l > 0 ? b.qoff[l - 1] : 0;
+__device__ __forceinline__ void push(const BatchDev &b, uint64_t *dst, int w, uint64_t F) {
+    const uint64_t m = F & ~b.vis[w];
+    if (m) atomicOr((unsigned long long *)&dst[w], (unsigned long long)m);  // result unused: no return
+}
+
+// Level l: Q_l = Q[qoff[l], ctr[0]).  Records qoff[l+1]; an empty Q_l ends the
+// forward with depth l-1.  Clears Q_{l-1} from mask array (l+2)%3.
+__global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
+    if (ldi(&b.ctr[3]) >= 0) return;
+    const int q0 = b.qoff[l];
+    const int q1 = min(ldi(&b.ctr[0]), b.qcap);
+    const bool stop = (q1 == q0) || ldi(&b.ctr[2]) != 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        b.qoff[l + 1] = q1;
+        if (stop) b.ctr[3] = l - 1;
+    }
+    if (stop) return;
     uint64_t *dst = mbuf(b, l + 1);
-    uint64_t *clr = mbuf(b, l + 2);
-    for (int i = qm0 + blockIdx.x * BB + threadIdx.x; i < q0; i += gridDim.x * BB) clr[b.Q[i]] = 0;
+    if (l > 0) {
+        uint64_t *clr = mbuf(b, l + 2);
+        for (int i = b.qoff[l - 1] + blockIdx.x * BB + threadIdx.x; i < q0; i += gridDim.x * BB) clr[b.Q[i]] = 0;
+    }
     const int lane = threadIdx.x & 63;
-    for (int e = q0 + gwave(); e < q1; e += nwaves()) {
-        const int v = b.Q[e];
-        const uint64_t F = b.Qm[e];
-        const int j0 = b.rp[v], j1 = b.rp[v + 1];
-        for (int c = j0; c < j1; c += 64) {
-            const int j = c + lane;
-            bool fresh = false;
-            int w = 0;
-            if (j < j1) {
-                w = b.col[j];
-                const uint64_t m = F & ~b.vis[w];
-                if (m) {
-                    const uint64_t old = atomicOr((unsigned long long *)&dst[w], (unsigned long long)m);
-                    fresh = (old == 0);
-                }
-            }
-            const uint64_t bal = __ballot(fresh);
-            if (bal) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(&b.ctr[0], __popcll(bal));
-                base = __shfl(base, 0, 64);
-                if (fresh) {
-                    const int pos = base + __popcll(bal & ((1ull << lane) - 1));
-                    if (pos < b.qcap) b.Q[pos] = w;
-                    else b.ctr[2] = 1;
-                }
-            }
+    for (int base = q0 + gwave() * 64; base < q1; base += nwaves() * 64) {
+        const int e = base + lane;
+        int j0 = 0, j1 = 0;
+        uint64_t F = 0;
+        if (e < q1) {
+            const int v = b.Q[e];
+            F = b.Qm[e];
+            j0 = b.rp[v];
+            j1 = b.rp[v + 1];
         }
+        const int d = j1 - j0;
+        if (d <= SMALL)
+            for (int j = j0; j < j1; ++j) push(b, dst, b.col[j], F);
+        uint64_t bal = __ballot(d > SMALL && !is_hub(b, d));
+        while (bal) {
+            const int src = __ffsll((long long)bal) - 1;
+            bal &= bal - 1;
+            const uint64_t FF = shfl64(F, src);
+            const int a0 = __shfl(j0, src, 64), a1 = __shfl(j1, src, 64);
+            for (int j = a0 + lane; j < a1; j += 64) push(b, dst, b.col[j], FF);
+        }
+    }
+    // hub slices: the hub's level-l lanes are its mask in mb[l%3]
+    const uint64_t *cur = mbuf(b, l);
+    for (int s = gwave(); s < b.n_slice; s += nwaves()) {
+        const int v = b.hub_v[b.sl_hub[s]];
+        const uint64_t F = cur[v];
+        if (!F) continue;
+        for (int j = b.sl_j0[s] + lane; j < b.sl_j1[s]; j += 64) push(b, dst, b.col[j], F);
     }
 }
 
 // --------------------------------------------------------------- pull ---
-// For w in Q_{l+1}: sigma over predecessors (lanes in mb[l][v]), CSR order.
 template <bool F64>
-__device__ void pull_level(const BatchDev &b, int l, int q1, int q2) {
-    const uint64_t *fr = mbuf(b, l);
-    const uint64_t *nx = mbuf(b, l + 1);
+__device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, uint64_t N, int a0, int a1,
+                                          uint64_t &acc, double &facc, bool &ovf) {
     const int lane = threadIdx.x & 63;
     const uint64_t me = 1ull << lane;
-    for (int e = q1 + gwave(); e < q2; e += nwaves()) {
-        const int w = b.Q[e];
-        const uint64_t N = nx[w];
-        const int j0 = b.rp[w], j1 = b.rp[w + 1];
+    for (int c = a0; c < a1; c += 64) {
+        const int j = c + lane;
+        int v = 0;
+        uint64_t f = 0;
+        if (j < a1) {
+            v = b.col[j];
+            f = fr[v] & N;
+        }
+        uint64_t bal = __ballot(f != 0);
+        while (bal) {
+            uint64_t x[UNR];
+            bool use[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                use[u] = false;
+                x[u] = 0;
+                if (bal) {
+                    const int src = __ffsll((long long)bal) - 1;
+                    bal &= bal - 1;
+                    const int vv = __shfl(v, src, 64);
+                    const uint64_t ff = shfl64(f, src);
+                    if (ff & me) {
+                        use[u] = true;
+                        x[u] = b.sig[(int64_t)vv * 64 + lane];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                if (!use[u]) continue;
+                if (F64) {
+                    facc += u2d(x[u]);
+                } else {
+                    const uint64_t s = acc + x[u];
+                    ovf |= (s < acc);
+                    acc = s;
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_flush(const BatchDev &b, int *sw, uint64_t *sm, int &cnt, int lane) {
+    if (cnt == 0) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&b.ctr[0], cnt);
+    base = __shfl(base, 0, 64);
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < cnt; i += 64) {
+        const int pos = base + i;
+        if (pos < b.qcap) {
+            b.Q[pos] = sw[i];
+            b.Qm[pos] = sm[i];
+        } else {
+            b.ctr[2] = 1;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    cnt = 0;
+}
+
+// Builds Q_{l+1} from the non-zero entries of mb[(l+1)%3] and their sigma.
+template <bool F64>
+__global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
+    __shared__ int s_w[NW][STAGE];
+    __shared__ uint64_t s_m[NW][STAGE];
+    if (ldi(&b.ctr[3]) >= 0) return;
+    const uint64_t *fr = mbuf(b, l);
+    const uint64_t *nx = mbuf(b, l + 1);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t me = 1ull << lane;
+    const bool split = !F64 && b.n_hub > 0;
+    int cnt = 0;  // wave-uniform
+    for (int base = gwave() * 64; base < b.n; base += nwaves() * 64) {
+        const int w0 = base + lane;
+        const uint64_t N0 = w0 < b.n ? nx[w0] : 0;
+        const uint64_t bal = __ballot(N0 != 0);
+        if (!bal) continue;
+        const int k0 = cnt;
+        if (N0) {
+            const int slot = k0 + __popcll(bal & (me - 1));
+            s_w[wv][slot] = w0;
+            s_m[wv][slot] = N0;
+        }
+        cnt += __popcll(bal);
+        __builtin_amdgcn_wave_barrier();
+        for (int k = k0; k < cnt; ++k) {
+            const int w = s_w[wv][k];
+            const uint64_t N = s_m[wv][k];
+            const int a0 = b.rp[w], a1 = b.rp[w + 1];
+            if (!(split && is_hub(b, a1 - a0))) {
+                uint64_t acc = 0;
+                double facc = 0.0;
+                bool ovf = false;
+                sigma_sum<F64>(b, fr, N, a0, a1, acc, facc, ovf);
+                if (N & me) {
+                    b.sig[(int64_t)w * 64 + lane] = F64 ? d2u(facc) : acc;
+                    if (!F64 && (ovf || (b.limit && acc >= b.limit))) b.ctr[1] = 1;
+                }
+            }
+            if ((N & me) && b.lvl) b.lvl[(int64_t)w * 64 + lane] = (uint16_t)(l + 1);
+            if (lane == 0) b.vis[w] |= N;
+        }
+        if (cnt >= STAGE_FLUSH) stage_flush(b, s_w[wv], s_m[wv], cnt, lane);
+    }
+    stage_flush(b, s_w[wv], s_m[wv], cnt, lane);
+    // hub slices: the level mask of every hub is recorded for the backward;
+    // partial sigma sums are added atomically (exact integers)
+    for (int s = gwave(); s < b.n_slice; s += nwaves()) {
+        const int h = b.sl_hub[s];
+        const int v = b.hub_v[h];
+        const uint64_t N = nx[v];
+        if (b.sl_j0[s] == b.rp[v] && lane == 0 && l + 1 < b.hub_stride)
+            b.hub_mask[(int64_t)h * b.hub_stride + l + 1] = N;
+        if (!split || !N) continue;
         uint64_t acc = 0;
         double facc = 0.0;
         bool ovf = false;
-        for (int c = j0; c < j1; c += 64) {
-            const int j = c + lane;
-            int v = 0;
-            uint64_t f = 0;
-            if (j < j1) {
-                v = b.col[j];
-                f = fr[v] & N;
-            }
-            uint64_t bal = __ballot(f != 0);
-            while (bal) {
-                uint64_t x[UNR];
-                bool use[UNR];
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-                    use[u] = false;
-                    x[u] = 0;
-                    if (bal) {
-                        const int src = __ffsll((long long)bal) - 1;
-                        bal &= bal - 1;
-                        const int vv = __shfl(v, src, 64);
-                        const uint64_t ff = shfl64(f, src);
-                        if (ff & me) {
-                            use[u] = true;
-                            x[u] = b.sig[(int64_t)vv * 64 + lane];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-                    if (!use[u]) continue;
-                    if (F64) {
-                        facc += u2d(x[u]);
-                    } else {
-                        const uint64_t s = acc + x[u];
-                        ovf |= (s < acc);
-                        acc = s;
-                    }
-                }
-            }
-        }
-        if (N & me) {
-            b.sig[(int64_t)w * 64 + lane] = F64 ? d2u(facc) : acc;
-            if (b.lvl) b.lvl[(int64_t)w * 64 + lane] = (uint16_t)(l + 1);
-            if (!F64 && (ovf || (b.limit && acc >= b.limit))) b.ctr[1] = 1;
-        }
-        if (lane == 0) {
-            b.vis[w] |= N;
-            b.Qm[e] = N;
+        sigma_sum<false>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf);
+        if ((N & me) && acc) {
+            const uint64_t old =
+                atomicAdd((unsigned long long *)&b.sig[(int64_t)v * 64 + lane], (unsigned long long)acc);
+            const uint64_t nw = old + acc;
+            if (ovf || nw < old || (b.limit && nw >= b.limit)) b.ctr[1] = 1;
         }
     }
-}
-
-// Level l of the forward is two launches: expand (all of Q_l's pushes must
-// land before any pull reads the masks of Q_{l+1}) then pull.  Both return at
-// once after the forward has ended (depth recorded in ctr[3]).
-__global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
-    if (__hip_atomic_load(&b.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= 0) return;
-    expand_level(b, l);
-}
-
-template <bool F64>
-__global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
-    if (__hip_atomic_load(&b.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= 0) return;
-    const int q1 = b.qoff[l + 1];
-    const int q2 = min(__hip_atomic_load(&b.ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), b.qcap);
-    const bool stop = (q2 == q1) || b.ctr[2] != 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        b.qoff[l + 2] = q2;
-        if (stop) b.ctr[3] = l;  // depth: Q_{l+1} is empty (or the lists overflowed)
-    }
-    if (stop) return;
-    pull_level<F64>(b, l, q1, q2);
 }
 
 // ----------------------------------------------------------- backward ---
@@ -222,6 +307,65 @@ struct SumD {
     __device__ __forceinline__ double value() const { return s; }
 };
 
+template <typename TT>
+__device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *succ, uint64_t M, int a0, int a1) {
+    using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
+    const TT *tt = (const TT *)b.tt;
+    const int lane = threadIdx.x & 63;
+    const uint64_t me = 1ull << lane;
+    Acc acc;
+    for (int c = a0; c < a1; c += 64) {
+        const int j = c + lane;
+        int w = 0;
+        uint64_t x = 0;
+        if (j < a1) {
+            w = b.col[j];
+            x = succ[w] & M;
+        }
+        uint64_t bal = __ballot(x != 0);
+        while (bal) {
+            TT t[UNR];
+            bool use[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                use[u] = false;
+                t[u] = 0;
+                if (bal) {
+                    const int src = __ffsll((long long)bal) - 1;
+                    bal &= bal - 1;
+                    const int ww = __shfl(w, src, 64);
+                    const uint64_t xx = shfl64(x, src);
+                    if (xx & me) {
+                        use[u] = true;
+                        t[u] = tt[(int64_t)ww * 64 + lane];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+                if (use[u]) acc.add(t[u]);
+        }
+    }
+    return acc.value();
+}
+
+// rho complete for (v, lanes M): delta, BC contribution and the term t_v.
+template <bool F64, typename TT>
+__device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t M, double rho) {
+    const int lane = threadIdx.x & 63;
+    double contrib = 0.0;
+    if (M & (1ull << lane)) {
+        const uint64_t sx = b.sig[(int64_t)v * 64 + lane];
+        const double sv = F64 ? u2d(sx) : (double)sx;
+        const double delta = sv * rho;
+        contrib = b.ws[lane] * delta;
+        ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + rho);
+        if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
+    }
+    contrib = wave_sum(contrib);
+    if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&b.bc[v], contrib);
+}
+
 // Level l <= depth: entries Q_l; Q_{l+2} is cleared from mb[(l+2)%3] and Q_l
 // scattered into mb[l%3] for level l-1.
 template <bool F64, typename TT>
@@ -229,7 +373,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     // the host launches the backward speculatively behind the forward: a
     // level beyond the depth, an unfinished forward (depth -1), a sigma
     // overflow (the tile is redone in FP64) or full lists all mean no work
-    const int D = __hip_atomic_load(&b.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int D = ldi(&b.ctr[3]);
     if (l > D || (!F64 && b.ctr[1]) || b.ctr[2]) return;
     const int q0 = b.qoff[l], q1 = b.qoff[l + 1];
     const int c0 = l + 2 <= D ? b.qoff[l + 2] : 0, c1 = l + 2 <= D ? b.qoff[l + 3] : 0;
@@ -237,60 +381,34 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     const uint64_t *succ = mbuf(b, l + 1);
     uint64_t *clr = mbuf(b, l + 2);
     for (int i = c0 + blockIdx.x * BB + threadIdx.x; i < c1; i += gridDim.x * BB) clr[b.Q[i]] = 0;
-    TT *tt = (TT *)b.tt;
     const int lane = threadIdx.x & 63;
-    const uint64_t me = 1ull << lane;
-    using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
     for (int e = q0 + gwave(); e < q1; e += nwaves()) {
         const int v = b.Q[e];
         const uint64_t M = b.Qm[e];
         if (lane == 0) own[v] = M;
-        const int j0 = b.rp[v], j1 = b.rp[v + 1];
-        Acc acc;
-        for (int c = j0; c < j1; c += 64) {
-            const int j = c + lane;
-            int w = 0;
-            uint64_t x = 0;
-            if (j < j1) {
-                w = b.col[j];
-                x = succ[w] & M;
-            }
-            uint64_t bal = __ballot(x != 0);
-            while (bal) {
-                TT t[UNR];
-                bool use[UNR];
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-                    use[u] = false;
-                    t[u] = 0;
-                    if (bal) {
-                        const int src = __ffsll((long long)bal) - 1;
-                        bal &= bal - 1;
-                        const int ww = __shfl(w, src, 64);
-                        const uint64_t xx = shfl64(x, src);
-                        if (xx & me) {
-                            use[u] = true;
-                            t[u] = tt[(int64_t)ww * 64 + lane];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < UNR; ++u)
-                    if (use[u]) acc.add(t[u]);
-            }
-        }
-        double contrib = 0.0;
-        if (M & me) {
-            const uint64_t sx = b.sig[(int64_t)v * 64 + lane];
-            const double sv = F64 ? u2d(sx) : (double)sx;
-            const double rho = acc.value();
-            const double delta = sv * rho;
-            contrib = b.ws[lane] * delta;
-            tt[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + rho);
-            if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
-        }
-        contrib = wave_sum(contrib);
-        if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&b.bc[v], contrib);
+        const int a0 = b.rp[v], a1 = b.rp[v + 1];
+        if (is_hub(b, a1 - a0) && l < b.hub_stride) continue;  // hub slices below
+        finish_vertex<F64, TT>(b, v, M, rho_sum<TT>(b, succ, M, a0, a1));
+    }
+    // hub slices (levels with recorded hub masks): partial rho into FP64
+    // accumulators; the last slice of a hub to finish (counter) completes it
+    if (l >= b.hub_stride) return;
+    for (int s = gwave(); s < b.n_slice; s += nwaves()) {
+        const int h = b.sl_hub[s];
+        const uint64_t M = b.hub_mask[(int64_t)h * b.hub_stride + l];
+        if (!M) continue;
+        const int v = b.hub_v[h];
+        const double part = rho_sum<TT>(b, succ, M, b.sl_j0[s], b.sl_j1[s]);
+        if ((M & (1ull << lane)) && part != 0.0) unsafeAtomicAdd(&b.hub_acc[h * 64 + lane], part);
+        __threadfence();
+        int done = 0;
+        if (lane == 0) done = atomicAdd(&b.hub_cnt[h], 1);
+        done = __shfl(done, 0, 64);
+        if (done != b.hub_nsl[h] - 1) continue;
+        __threadfence();
+        const double rho = __hip_atomic_load(&b.hub_acc[h * 64 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        finish_vertex<F64, TT>(b, v, M, rho);
+        if (lane == 0) b.hub_cnt[h] = 0;
     }
 }
 

@@ -4,6 +4,12 @@
 // tile with FP64 sigma.  Sources are ordered by a BFS from the component's
 // highest-degree vertex so the 64 sources of a tile are close together and
 // their frontiers overlap (fewer (vertex, level) list entries per source).
+//
+// Several tiles run at once, one per slot, each slot with its own stream and
+// state: the many small levels of one tile (near its sources, along long
+// chains) leave most compute units idle, and the other slots fill them.  A
+// slot enqueues a tile as one fixed launch sequence and is polled through an
+// event; the host never waits on a single tile.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -13,36 +19,47 @@
 
 using namespace bcx;
 
+// Hub vertices of a component: degree above HUB_DEG, arcs cut into slices
+// of HUB_SLICE so several waves share one hub in every level kernel.
+constexpr int HUB_DEG = 256, HUB_SLICE = 256, HUB_LEVELS = 1024;
+constexpr int MAX_SLOTS = 4;
+
+struct Slot {
+    DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, hub_mask, hub_acc, hub_cnt, lvl, dbg;
+    int32_t n_cap = 0, qcap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int32_t *pinned = nullptr;
+    // tile in flight
+    int tile = -1, ns = 0, L = 0, C = 0, attempts = 0;
+    bool f64 = false;
+};
+
 struct BatchedState {
-    DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, srcs, ws, lvl, dbg;
-    int32_t n_cap = 0;
-    int32_t qcap = 0;
-    int32_t src_cap = 0;
+    Slot slots[MAX_SLOTS];
+    int n_slots = 0;
+    DBuf srcs, ws, hub_v, hub_nsl, sl_hub, sl_j0, sl_j1;
+    int32_t src_cap = 0, n_hub = 0, n_slice = 0, hub_stride = 0;
     bool probe = false;
-    int32_t *pinned = nullptr;  // ctr read-back
     int last_depth = 8;
 };
 
-static int bs_ensure(bcx_ctx *ctx, int32_t n, int32_t n_src, bool probe) {
-    if (!ctx->batched) ctx->batched = new BatchedState();
-    BatchedState &s = *ctx->batched;
+static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
     int rc;
     const size_t N = (size_t)n;
-    if (!s.pinned) {
-        if (hipHostMalloc((void **)&s.pinned, 64, hipHostMallocDefault) != hipSuccess) {
-            s.pinned = nullptr;
-            return ctx_fail(ctx, BCX_E_HIP, "pinned counters");
-        }
+    if (!s.stream) {
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void **)&s.pinned, 64, hipHostMallocDefault) != hipSuccess)
+            return ctx_fail(ctx, BCX_E_HIP, "batched slot stream");
     }
     if (n > s.n_cap) {
-        if ((rc = s.vis.ensure(ctx, 8 * N, "batched vis")) ||
-            (rc = s.mb.ensure(ctx, 24 * N, "batched masks")) ||
+        if ((rc = s.vis.ensure(ctx, 8 * N, "batched vis")) || (rc = s.mb.ensure(ctx, 24 * N, "batched masks")) ||
             (rc = s.sig.ensure(ctx, 8 * 64 * N, "batched sigma")) ||
             (rc = s.tt.ensure(ctx, 8 * 64 * N, "batched dependency terms")) ||
             (rc = s.qoff.ensure(ctx, 4 * (N + 8), "batched level offsets")))
             return rc;
-        rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, ctx->stream), "memset");
-        if (rc) return rc;
+        if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, s.stream), "memset"))) return rc;
         s.n_cap = n;
     }
     if (s.qcap < 8 * n + 4096) {
@@ -52,23 +69,36 @@ static int bs_ensure(bcx_ctx *ctx, int32_t n, int32_t n_src, bool probe) {
             return rc;
         s.qcap = (int32_t)q;
     }
-    if (n_src > s.src_cap) {
-        if ((rc = s.srcs.ensure(ctx, 4 * (size_t)n_src, "batched sources")) ||
-            (rc = s.ws.ensure(ctx, 8 * (size_t)n_src, "batched weights")))
-            return rc;
-        s.src_cap = n_src;
-    }
     if ((rc = s.ctr.ensure(ctx, 64, "batched counters"))) return rc;
-    if (probe) {
-        if ((rc = s.lvl.ensure(ctx, 2 * 64 * N, "probe levels")) || (rc = s.dbg.ensure(ctx, 8 * 64 * N, "probe delta")))
-            return rc;
-    }
-    s.probe = probe;
+    if (probe && ((rc = s.lvl.ensure(ctx, 2 * 64 * N, "probe levels")) ||
+                  (rc = s.dbg.ensure(ctx, 8 * 64 * N, "probe delta"))))
+        return rc;
     return BCX_OK;
 }
 
-static BatchDev make_dev(bcx_ctx *ctx, const DComp &dc, double *bc, uint64_t limit, int tile) {
-    BatchedState &s = *ctx->batched;
+// Slots sized for n; as many as fit in a quarter of the free capacity.
+static int bs_ensure(bcx_ctx *ctx, int32_t n, int32_t n_src, int want_slots, bool probe) {
+    if (!ctx->batched) ctx->batched = new BatchedState();
+    BatchedState &bs = *ctx->batched;
+    const size_t per = (size_t)n * (8 + 24 + 512 + 512 + 8 * 12) + (1 << 20);
+    const size_t room = ctx->mem_cap > ctx->mem_used ? (ctx->mem_cap - ctx->mem_used) : 0;
+    int fit = (int)std::max<size_t>(1, std::min<size_t>(MAX_SLOTS, room / 2 / std::max<size_t>(per, 1)));
+    bs.n_slots = std::max(1, std::min(want_slots, fit));
+    int rc;
+    for (int i = 0; i < bs.n_slots; ++i)
+        if ((rc = slot_ensure(ctx, bs.slots[i], n, probe))) return rc;
+    if (n_src > bs.src_cap) {
+        if ((rc = bs.srcs.ensure(ctx, 4 * (size_t)n_src, "batched sources")) ||
+            (rc = bs.ws.ensure(ctx, 8 * (size_t)n_src, "batched weights")))
+            return rc;
+        bs.src_cap = n_src;
+    }
+    bs.probe = probe;
+    return BCX_OK;
+}
+
+static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uint64_t limit) {
+    BatchedState &bs = *ctx->batched;
     BatchDev b{};
     b.n = dc.n;
     b.rp = dc.dev.rp;
@@ -80,81 +110,205 @@ static BatchDev make_dev(bcx_ctx *ctx, const DComp &dc, double *bc, uint64_t lim
     b.mb2 = b.mb1 + s.n_cap;
     b.sig = (uint64_t *)s.sig.p;
     b.tt = s.tt.p;
-    b.lvl = s.probe ? (uint16_t *)s.lvl.p : nullptr;
-    b.dbg_delta = s.probe ? (double *)s.dbg.p : nullptr;
+    b.lvl = bs.probe ? (uint16_t *)s.lvl.p : nullptr;
+    b.dbg_delta = bs.probe ? (double *)s.dbg.p : nullptr;
     b.Q = (int32_t *)s.Q.p;
     b.Qm = (uint64_t *)s.Qm.p;
     b.ctr = (int32_t *)s.ctr.p;
     b.qoff = (int32_t *)s.qoff.p;
     b.qcap = s.qcap;
-    b.srcs = (const int32_t *)s.srcs.p + (size_t)tile * BATCH_W;
-    b.ws = (const double *)s.ws.p + (size_t)tile * BATCH_W;
+    b.n_hub = bs.n_hub;
+    b.n_slice = bs.n_slice;
+    b.hub_deg = HUB_DEG;
+    b.hub_stride = bs.hub_stride;
+    b.hub_v = (const int32_t *)bs.hub_v.p;
+    b.hub_nsl = (const int32_t *)bs.hub_nsl.p;
+    b.sl_hub = (const int32_t *)bs.sl_hub.p;
+    b.sl_j0 = (const int32_t *)bs.sl_j0.p;
+    b.sl_j1 = (const int32_t *)bs.sl_j1.p;
+    b.hub_mask = (uint64_t *)s.hub_mask.p;
+    b.hub_acc = (double *)s.hub_acc.p;
+    b.hub_cnt = (int32_t *)s.hub_cnt.p;
+    b.srcs = (const int32_t *)bs.srcs.p + (size_t)s.tile * BATCH_W;
+    b.ws = (const double *)bs.ws.p + (size_t)s.tile * BATCH_W;
     b.bc = bc;
     b.limit = limit;
     return b;
 }
 
-// One tile: init, forward levels and a speculative backward launched as one
-// sequence, then a single read-back of the counters.  A forward deeper than
-// the launched levels continues from where it stopped; a sigma overflow
-// (uint64) or full level lists leave the BC untouched and redo the tile.
-static int run_tile(bcx_ctx *ctx, const DComp &dc, int tile, int ns, bool fp32, double *bc, uint64_t limit,
-                    bcx_stats *st, bool *f64_out) {
-    BatchedState &s = *ctx->batched;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)dc.n + 3) / 4, (int64_t)ctx->n_cu * 8));
-    bool f64 = false;
+struct TileJob {
+    const DComp *dc;
+    double *bc;
+    uint64_t limit;
+    bool fp32;
+    int grid;
+};
+
+// Forward levels [L, L+C), the speculative backward L+C-1 .. 1 (kernels past
+// the depth or of an unfinished forward return at once), the cleanup, the
+// counter read-back and the completion event.
+static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st) {
+    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, s.f64 ? 0 : j.limit);
     int rc;
-    for (int attempt = 0; attempt < 6; ++attempt) {
-        BatchDev b = make_dev(ctx, dc, bc, f64 ? 0 : limit, tile);
-        ctx->timer.begin(ctx);
-        if ((rc = ctx_hip(ctx, launch_b_init(b, ns, f64, grid, ctx->stream), "init"))) return rc;
-        int L = 0, C = std::max(4, s.last_depth + 3);
-        int32_t *h = s.pinned;
-        int launches = 2;
-        while (true) {
-            for (int l = L; l < L + C; ++l)
-                if ((rc = ctx_hip(ctx, launch_b_forward(b, f64, l, grid, ctx->stream), "forward"))) return rc;
-            L += C;
-            for (int l = L - 1; l >= 1; --l)
-                if ((rc = ctx_hip(ctx, launch_b_back(b, f64, fp32, l, grid, ctx->stream), "back"))) return rc;
-            if ((rc = ctx_hip(ctx, launch_b_finish(b, grid, ctx->stream), "finish"))) return rc;
-            launches += 2 * C + (L - 1) + 1;
-            if ((rc = ctx_hip(ctx, hipMemcpyAsync(h, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
-                              "counters")) ||
-                (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "batched kernels")))
-                return rc;
-            if (h[3] >= 0) break;
-            C = std::min(2 * C, 4096);
-        }
-        ctx->timer.end(ctx);
-        if (st) st->n_launches += launches;
-        const int D = h[3];
-        if (h[2]) {
-            // lists full: masks may hold entries of vertices never listed
-            if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, ctx->stream), "memset"))) return rc;
-            const int64_t q = std::min<int64_t>((int64_t)s.qcap * 4, std::min<int64_t>((int64_t)64 * dc.n + 4096,
-                                                                                        INT32_MAX / 2));
-            if (q <= s.qcap) return ctx_fail(ctx, BCX_E_NOMEM, "batched level lists cannot grow");
-            if ((rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "sync")) ||
-                (rc = s.Q.ensure(ctx, 4 * (size_t)q, "batched lists")) ||
-                (rc = s.Qm.ensure(ctx, 8 * (size_t)q, "batched lists")))
-                return rc;
-            s.qcap = (int32_t)q;
-            continue;
-        }
-        if (!f64 && h[1]) {
-            if ((rc = ctx_hip(ctx, launch_b_clear_all(b, std::min(h[0], s.qcap), grid, ctx->stream), "clear")))
-                return rc;
-            f64 = true;
-            if (st) st->n_f64_sigma += ns;
-            continue;
-        }
-        s.last_depth = D;
-        if (st && D > st->max_level) st->max_level = D;
-        if (f64_out) *f64_out = f64;
-        return BCX_OK;
+    for (int l = s.L; l < s.L + s.C; ++l)
+        if ((rc = ctx_hip(ctx, launch_b_forward(b, s.f64, l, j.grid, s.stream), "forward"))) return rc;
+    s.L += s.C;
+    for (int l = s.L - 1; l >= 1; --l)
+        if ((rc = ctx_hip(ctx, launch_b_back(b, s.f64, j.fp32, l, j.grid, s.stream), "back"))) return rc;
+    if ((rc = ctx_hip(ctx, launch_b_finish(b, j.grid, s.stream), "finish"))) return rc;
+    if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1;
+    if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.pinned, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream),
+                      "counters")))
+        return rc;
+    return ctx_hip(ctx, hipEventRecord(s.done, s.stream), "event");
+}
+
+static int start_tile(bcx_ctx *ctx, Slot &s, int tile, int ns, bool f64, const TileJob &j, bcx_stats *st) {
+    s.tile = tile;
+    s.ns = ns;
+    s.f64 = f64;
+    s.L = 0;
+    s.C = std::max(4, ctx->batched->last_depth + 3);
+    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, f64 ? 0 : j.limit);
+    int rc = ctx_hip(ctx, launch_b_init(b, ns, f64, j.grid, s.stream), "init");
+    if (rc) return rc;
+    if (st) st->n_launches += 2;
+    return enqueue_chunk(ctx, s, j, st);
+}
+
+// Returns 1 while the slot's tile is running, 0 when it has completed.
+static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int *rc_out) {
+    *rc_out = BCX_OK;
+    hipError_t q = hipEventQuery(s.done);
+    if (q == hipErrorNotReady) return 1;
+    if (q != hipSuccess) { *rc_out = ctx_hip(ctx, q, "batched tile"); return 0; }
+    const int32_t *h = s.pinned;
+    int rc;
+    if (h[3] < 0) {  // deeper than the levels launched so far
+        s.C = std::min(2 * s.C, 4096);
+        *rc_out = enqueue_chunk(ctx, s, j, st);
+        return *rc_out ? 0 : 1;
     }
-    return ctx_fail(ctx, BCX_E_OVERFLOW, "batched tile could not be completed");
+    if (++s.attempts > 6) { *rc_out = ctx_fail(ctx, BCX_E_OVERFLOW, "batched tile could not be completed"); return 0; }
+    if (h[2]) {
+        // lists full: masks may hold entries of vertices never listed
+        const int64_t q2 = std::min<int64_t>((int64_t)s.qcap * 4, std::min<int64_t>((int64_t)64 * j.dc->n + 4096,
+                                                                                     INT32_MAX / 2));
+        if (q2 <= s.qcap) { *rc_out = ctx_fail(ctx, BCX_E_NOMEM, "batched level lists cannot grow"); return 0; }
+        if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, s.stream), "memset")) ||
+            (rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "sync")) ||
+            (rc = s.Q.ensure(ctx, 4 * (size_t)q2, "batched lists")) ||
+            (rc = s.Qm.ensure(ctx, 8 * (size_t)q2, "batched lists"))) {
+            *rc_out = rc;
+            return 0;
+        }
+        s.qcap = (int32_t)q2;
+        *rc_out = start_tile(ctx, s, s.tile, s.ns, s.f64, j, st);
+        return *rc_out ? 0 : 1;
+    }
+    if (!s.f64 && h[1]) {
+        // uint64 sigma overflowed: the backward did not run; redo in FP64
+        BatchDev b = make_dev(ctx, s, *j.dc, j.bc, 0);
+        if ((rc = ctx_hip(ctx, launch_b_clear_all(b, std::min(h[0], s.qcap), j.grid, s.stream), "clear"))) {
+            *rc_out = rc;
+            return 0;
+        }
+        if (st) st->n_f64_sigma += s.ns;
+        *rc_out = start_tile(ctx, s, s.tile, s.ns, true, j, st);
+        return *rc_out ? 0 : 1;
+    }
+    ctx->batched->last_depth = h[3];
+    if (st && h[3] > st->max_level) st->max_level = h[3];
+    return 0;
+}
+
+// Runs tiles [0, ntiles) over the slots; kernel time is the span between
+// events on the context stream that every slot stream is ordered against.
+static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob &j, bcx_stats *st,
+                     bool *f64_out) {
+    BatchedState &bs = *ctx->batched;
+    const int64_t ntiles = (n_src + BATCH_W - 1) / BATCH_W;
+    const int ns_slots = (int)std::min<int64_t>(bs.n_slots, ntiles);
+    int rc;
+    hipEvent_t e0;
+    if ((rc = ctx_hip(ctx, hipEventCreateWithFlags(&e0, hipEventDisableTiming), "event"))) return rc;
+    ctx->timer.begin(ctx);
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int i = 0; i < ns_slots; ++i) (void)hipStreamWaitEvent(bs.slots[i].stream, e0, 0);
+    int64_t next = 0;
+    int active = 0;
+    bool any_f64 = false;
+    for (int i = 0; i < ns_slots; ++i) bs.slots[i].tile = -1;
+    rc = BCX_OK;
+    while ((next < ntiles || active > 0) && !rc) {
+        for (int i = 0; i < ns_slots && !rc; ++i) {
+            Slot &s = bs.slots[i];
+            if (s.tile < 0) {
+                if (next >= ntiles) continue;
+                s.attempts = 0;
+                const int ns = (int)std::min<int64_t>(BATCH_W, n_src - next * BATCH_W);
+                rc = start_tile(ctx, s, (int)next++, ns, false, j, st);
+                active++;
+                continue;
+            }
+            int r2;
+            if (poll_tile(ctx, s, j, st, &r2) == 0) {
+                rc = r2;
+                any_f64 |= s.f64;
+                s.tile = -1;
+                active--;
+            }
+        }
+    }
+    for (int i = 0; i < ns_slots; ++i) {
+        (void)hipEventRecord(bs.slots[i].done, bs.slots[i].stream);
+        (void)hipStreamWaitEvent(ctx->stream, bs.slots[i].done, 0);
+    }
+    ctx->timer.end(ctx);
+    for (int i = 0; i < ns_slots; ++i) (void)hipStreamSynchronize(bs.slots[i].stream);
+    (void)hipEventDestroy(e0);
+    if (f64_out) *f64_out = any_f64;
+    return rc;
+}
+
+// Static hub slice tables of one component.
+static int stage_hubs(bcx_ctx *ctx, const DComp &dc) {
+    BatchedState &bs = *ctx->batched;
+    const HostComp &hc = dc.host;
+    std::vector<int32_t> hv, nsl, slh, j0, j1;
+    for (int32_t v = 0; v < hc.n; ++v) {
+        const int32_t a = hc.rp[v], e = hc.rp[v + 1];
+        if (e - a <= HUB_DEG) continue;
+        const int32_t h = (int32_t)hv.size();
+        hv.push_back(v);
+        int32_t k = 0;
+        for (int32_t x = a; x < e; x += HUB_SLICE, ++k) {
+            slh.push_back(h);
+            j0.push_back(x);
+            j1.push_back(std::min(e, x + HUB_SLICE));
+        }
+        nsl.push_back(k);
+    }
+    const int64_t nh = (int64_t)hv.size();
+    bs.n_hub = (int32_t)nh;
+    bs.n_slice = (int32_t)slh.size();
+    bs.hub_stride = nh ? (int32_t)std::min<int64_t>(HUB_LEVELS, (int64_t)hc.n + 2) : 0;
+    if (!nh) return BCX_OK;
+    int rc;
+    if ((rc = upload(ctx, hv.data(), 4 * hv.size(), bs.hub_v, "hubs")) ||
+        (rc = upload(ctx, nsl.data(), 4 * nsl.size(), bs.hub_nsl, "hubs")) ||
+        (rc = upload(ctx, slh.data(), 4 * slh.size(), bs.sl_hub, "hub slices")) ||
+        (rc = upload(ctx, j0.data(), 4 * j0.size(), bs.sl_j0, "hub slices")) ||
+        (rc = upload(ctx, j1.data(), 4 * j1.size(), bs.sl_j1, "hub slices")))
+        return rc;
+    for (int i = 0; i < bs.n_slots; ++i) {
+        Slot &s = bs.slots[i];
+        if ((rc = s.hub_mask.ensure(ctx, 8 * (size_t)nh * bs.hub_stride, "hub masks")) ||
+            (rc = s.hub_acc.ensure(ctx, 8 * 64 * (size_t)nh, "hub accumulators")) ||
+            (rc = s.hub_cnt.ensure(ctx, 4 * (size_t)nh, "hub counters")))
+            return rc;
+    }
+    return ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "hubs");
 }
 
 static int stage_sources(bcx_ctx *ctx, const DComp &dc, const std::vector<int32_t> &srcs) {
@@ -165,11 +319,11 @@ static int stage_sources(bcx_ctx *ctx, const DComp &dc, const std::vector<int32_
         hs[i] = srcs[i];
         hw[i] = (double)dc.host.reach[srcs[i]] + 1.0;
     }
-    BatchedState &s = *ctx->batched;
+    BatchedState &bs = *ctx->batched;
     int rc;
-    if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.srcs.p, hs.data(), 4 * hs.size(), hipMemcpyHostToDevice, ctx->stream),
+    if ((rc = ctx_hip(ctx, hipMemcpyAsync(bs.srcs.p, hs.data(), 4 * hs.size(), hipMemcpyHostToDevice, ctx->stream),
                       "sources")) ||
-        (rc = ctx_hip(ctx, hipMemcpyAsync(s.ws.p, hw.data(), 8 * hw.size(), hipMemcpyHostToDevice, ctx->stream),
+        (rc = ctx_hip(ctx, hipMemcpyAsync(bs.ws.p, hw.data(), 8 * hw.size(), hipMemcpyHostToDevice, ctx->stream),
                       "weights")))
         return rc;
     return ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "sources");
@@ -196,6 +350,17 @@ static std::vector<int32_t> bfs_rank(const HostComp &hc) {
     return rank;
 }
 
+static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double *bc) {
+    TileJob j;
+    j.dc = &dc;
+    j.bc = bc;
+    j.limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
+                  ? (1ull << o->sigma_f64_threshold_log2) : 0;
+    j.fp32 = o->precision == BCX_PREC_FP32C;
+    j.grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)dc.n + 3) / 4, (int64_t)ctx->n_cu * 8));
+    return j;
+}
+
 int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, const bcx_opts *o, double *bc,
                       bcx_stats *st) {
     // every requested source runs as a BFS lane: the plan's BFS sources plus
@@ -210,18 +375,12 @@ int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, c
     const std::vector<int32_t> rank = bfs_rank(dc.host);
     std::sort(srcs.begin(), srcs.end(), [&](int32_t a, int32_t b) { return rank[a] < rank[b]; });
     srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
-    const size_t ntiles = (srcs.size() + BATCH_W - 1) / BATCH_W;
-    int rc = bs_ensure(ctx, dc.n, (int32_t)(ntiles * BATCH_W), false);
+    const int64_t ntiles = ((int64_t)srcs.size() + BATCH_W - 1) / BATCH_W;
+    const int want = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width)) : MAX_SLOTS;
+    int rc = bs_ensure(ctx, dc.n, (int32_t)(ntiles * BATCH_W), want, false);
     if (rc) return rc;
-    if ((rc = stage_sources(ctx, dc, srcs))) return rc;
-    const uint64_t limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
-                               ? (1ull << o->sigma_f64_threshold_log2) : 0;
-    const bool fp32 = o->precision == BCX_PREC_FP32C;
-    for (size_t t = 0; t < ntiles; ++t) {
-        const int ns = (int)std::min<size_t>(BATCH_W, srcs.size() - t * BATCH_W);
-        if ((rc = run_tile(ctx, dc, (int)t, ns, fp32, bc, limit, st, nullptr))) return rc;
-    }
-    return BCX_OK;
+    if ((rc = stage_sources(ctx, dc, srcs)) || (rc = stage_hubs(ctx, dc))) return rc;
+    return run_tiles(ctx, dc, (int64_t)srcs.size(), make_job(ctx, dc, o, bc), st, nullptr);
 }
 
 int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o, int32_t *level,
@@ -231,29 +390,27 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
     if (rc) return rc;
     DComp &dc = g->comps[0];
     const int32_t n = dc.n;
-    rc = bs_ensure(ctx, n, BATCH_W, true);
+    rc = bs_ensure(ctx, n, BATCH_W, 1, true);
     if (!rc) rc = ctx->bc_acc.ensure(ctx, sizeof(double) * n, "bc accumulator");
     bool f64 = false;
     std::vector<uint16_t> lv((size_t)n * 64);
     std::vector<uint64_t> sg((size_t)n * 64);
     std::vector<double> dl((size_t)n * 64);
     if (!rc) {
-        const uint64_t limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
-                                   ? (1ull << o->sigma_f64_threshold_log2) : 0;
         std::vector<int32_t> tile{src};
         rc = stage_sources(ctx, dc, tile);
-        if (!rc)
-            rc = run_tile(ctx, dc, 0, 1, o->precision == BCX_PREC_FP32C, (double *)ctx->bc_acc.p, limit, nullptr,
-                          &f64);
+        if (!rc) rc = stage_hubs(ctx, dc);
+        if (!rc) rc = run_tiles(ctx, dc, 1, make_job(ctx, dc, o, (double *)ctx->bc_acc.p), nullptr, &f64);
     }
-    // copies are ordered on the context stream (the kernels run there)
-    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), ctx->batched->lvl.p, 2 * lv.size(), hipMemcpyDeviceToHost,
-                                              ctx->stream), "copy");
-    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), ctx->batched->sig.p, 8 * sg.size(), hipMemcpyDeviceToHost,
-                                              ctx->stream), "copy");
-    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), ctx->batched->dbg.p, 8 * dl.size(), hipMemcpyDeviceToHost,
-                                              ctx->stream), "copy");
-    if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "copy");
+    Slot &s = ctx->batched->slots[0];
+    // copies are ordered on the slot stream the kernels ran on
+    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), s.lvl.p, 2 * lv.size(), hipMemcpyDeviceToHost, s.stream),
+                          "copy");
+    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), s.sig.p, 8 * sg.size(), hipMemcpyDeviceToHost, s.stream),
+                          "copy");
+    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), s.dbg.p, 8 * dl.size(), hipMemcpyDeviceToHost, s.stream),
+                          "copy");
+    if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "copy");
     (void)ctx->timer.collect();
     ctx->batched->probe = false;
     bcx_dgraph_free(ctx, g);
@@ -273,10 +430,17 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
 
 void batched_release(bcx_ctx *ctx) {
     if (!ctx->batched) return;
-    BatchedState &s = *ctx->batched;
-    for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.qoff, &s.srcs, &s.ws, &s.lvl, &s.dbg})
-        b->release(ctx);
-    if (s.pinned) (void)hipHostFree(s.pinned);
+    BatchedState &bs = *ctx->batched;
+    for (Slot &s : bs.slots) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.qoff, &s.hub_mask, &s.hub_acc,
+                        &s.hub_cnt, &s.lvl, &s.dbg})
+            b->release(ctx);
+        if (s.pinned) (void)hipHostFree(s.pinned);
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+    }
+    for (DBuf *b : {&bs.srcs, &bs.ws, &bs.hub_v, &bs.hub_nsl, &bs.sl_hub, &bs.sl_j0, &bs.sl_j1}) b->release(ctx);
     delete ctx->batched;
     ctx->batched = nullptr;
 }
