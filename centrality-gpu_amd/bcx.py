@@ -264,11 +264,15 @@ class Host:
         return f.value, e.value, x.value
 
 
+FLAG_NO_SMALL_KERNEL = 1
+
+
 def make_opts(path=PATH_PLAIN, precision=PREC_FP64, use_chains=True, subset=None,
-              row_cap_bytes=0, sigma_threshold_log2=0, batch_width=0):
+              row_cap_bytes=0, sigma_threshold_log2=0, batch_width=0, flags=0):
     o = Opts()
     o.path, o.precision, o.use_chains = path, precision, int(use_chains)
     o.batch_width = batch_width
+    o.flags = flags
     o.row_cap_bytes = row_cap_bytes
     o.sigma_f64_threshold_log2 = sigma_threshold_log2
     keep = None

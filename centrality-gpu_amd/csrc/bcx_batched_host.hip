@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "bcx_batched.h"
+#include "bcx_small.h"
 #include "bcx_runtime.h"
 
 using namespace bcx;
@@ -361,10 +362,9 @@ static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double
     return j;
 }
 
-int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, const bcx_opts *o, double *bc,
-                      bcx_stats *st) {
-    // every requested source runs as a BFS lane: the plan's BFS sources plus
-    // its chain sources
+// Every requested source of the plan runs as a BFS lane (its BFS sources and
+// its chain sources), ordered by BFS rank from the highest-degree vertex.
+std::vector<int32_t> batched_sources(const DComp &dc, const Plan &plan) {
     std::vector<int32_t> srcs(plan.free_src);
     for (auto &b : plan.batches) {
         for (size_t i = 0; i < b.bsrc.size(); ++i)
@@ -375,12 +375,135 @@ int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, c
     const std::vector<int32_t> rank = bfs_rank(dc.host);
     std::sort(srcs.begin(), srcs.end(), [&](int32_t a, int32_t b) { return rank[a] < rank[b]; });
     srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
+    return srcs;
+}
+
+int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, const bcx_opts *o, double *bc,
+                      bcx_stats *st) {
+    const std::vector<int32_t> srcs = batched_sources(dc, plan);
     const int64_t ntiles = ((int64_t)srcs.size() + BATCH_W - 1) / BATCH_W;
     const int want = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width)) : MAX_SLOTS;
     int rc = bs_ensure(ctx, dc.n, (int32_t)(ntiles * BATCH_W), want, false);
     if (rc) return rc;
     if ((rc = stage_sources(ctx, dc, srcs)) || (rc = stage_hubs(ctx, dc))) return rc;
     return run_tiles(ctx, dc, (int64_t)srcs.size(), make_job(ctx, dc, o, bc), st, nullptr);
+}
+
+// ------------------------------------------------------------------------
+// small components: one persistent launch over (component, 64-source) tiles
+// ------------------------------------------------------------------------
+struct SmallState {
+    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm;
+    int grid = 0;
+};
+
+int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+                  bcx_stats *st) {
+    if (jobs.empty()) return BCX_OK;
+    if (!ctx->batched) ctx->batched = new BatchedState();
+    if (!ctx->small) ctx->small = new SmallState();
+    SmallState &ss = *ctx->small;
+    std::vector<SmallComp> comps;
+    std::vector<SmallTile> tiles;
+    std::vector<int32_t> srcs;
+    std::vector<double> ws;
+    for (const SmallJob &jb : jobs) {
+        const DComp &dc = *jb.dc;
+        SmallComp c{};
+        c.n = dc.n;
+        c.rp = dc.dev.rp;
+        c.col = dc.dev.col;
+        c.reach = dc.dev.reach;
+        c.bc = bc_base + dc.offset;
+        const int32_t ci = (int32_t)comps.size();
+        comps.push_back(c);
+        for (size_t i = 0; i < jb.srcs.size(); i += BATCH_W) {
+            SmallTile t{};
+            t.comp = ci;
+            t.src0 = (int32_t)srcs.size();
+            t.ns = (int32_t)std::min<size_t>(BATCH_W, jb.srcs.size() - i);
+            tiles.push_back(t);
+            for (int k = 0; k < t.ns; ++k) {
+                srcs.push_back(jb.srcs[i + k]);
+                ws.push_back((double)dc.host.reach[jb.srcs[i + k]] + 1.0);
+            }
+        }
+    }
+    int rc;
+    if ((rc = upload(ctx, comps.data(), sizeof(SmallComp) * comps.size(), ss.comps, "small comps")) ||
+        (rc = upload(ctx, tiles.data(), sizeof(SmallTile) * tiles.size(), ss.tiles, "small tiles")) ||
+        (rc = upload(ctx, srcs.data(), 4 * srcs.size(), ss.srcs, "small sources")) ||
+        (rc = upload(ctx, ws.data(), 8 * ws.size(), ss.ws, "small weights")) ||
+        (rc = ss.counter.ensure(ctx, 64, "small counters")) ||
+        (rc = ss.redo.ensure(ctx, 4 * tiles.size(), "small redo list")))
+        return rc;
+    // two workgroups per CU (56 KB of LDS each), never more than the tiles
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), 2 * (int64_t)ctx->n_cu));
+    const size_t slab = (size_t)SMALL_N * 64;
+    if ((rc = ss.sig.ensure(ctx, 8 * slab * grid, "small sigma")) ||
+        (rc = ss.tt.ensure(ctx, 8 * slab * grid, "small dependency terms")) ||
+        (rc = ss.Q.ensure(ctx, 4 * slab * grid, "small lists")) ||
+        (rc = ss.Qm.ensure(ctx, 8 * slab * grid, "small lists")))
+        return rc;
+    SmallArgs a{};
+    a.comps = (const SmallComp *)ss.comps.p;
+    a.tiles = (const SmallTile *)ss.tiles.p;
+    a.n_tiles = (int32_t)tiles.size();
+    a.srcs = (const int32_t *)ss.srcs.p;
+    a.ws = (const double *)ss.ws.p;
+    a.counter = (int32_t *)ss.counter.p;
+    a.redo = (int32_t *)ss.redo.p;
+    a.sig = (uint64_t *)ss.sig.p;
+    a.tt = ss.tt.p;
+    a.Q = (int32_t *)ss.Q.p;
+    a.Qm = (uint64_t *)ss.Qm.p;
+    a.slab = (int64_t)slab;
+    a.limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
+                  ? (1ull << o->sigma_f64_threshold_log2) : 0;
+    const bool fp32 = o->precision == BCX_PREC_FP32C;
+    ctx->timer.begin(ctx);
+    if ((rc = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "small reset")) ||
+        (rc = ctx_hip(ctx, launch_small(a, false, fp32, grid, ctx->stream), "small")))
+        return rc;
+    ctx->timer.end(ctx);
+    if (st) st->n_launches += 2;
+    int32_t h[2];
+    if ((rc = ctx_hip(ctx, hipMemcpyAsync(h, a.counter, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "counter")) ||
+        (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel")))
+        return rc;
+    if (h[1] > 0) {
+        // tiles whose uint64 sigma overflowed added nothing: rerun them in FP64
+        std::vector<int32_t> redo((size_t)h[1]);
+        if ((rc = ctx_hip(ctx, hipMemcpyAsync(redo.data(), a.redo, 4 * redo.size(), hipMemcpyDeviceToHost,
+                                              ctx->stream), "redo")) ||
+            (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "redo")))
+            return rc;
+        std::vector<SmallTile> again;
+        for (int32_t t : redo) again.push_back(tiles[t]);
+        std::sort(again.begin(), again.end(), [](const SmallTile &x, const SmallTile &y) { return x.src0 < y.src0; });
+        if ((rc = upload(ctx, again.data(), sizeof(SmallTile) * again.size(), ss.tiles, "small tiles"))) return rc;
+        a.tiles = (const SmallTile *)ss.tiles.p;
+        a.n_tiles = (int32_t)again.size();
+        a.limit = 0;
+        if (st) for (auto &t : again) st->n_f64_sigma += t.ns;
+        ctx->timer.begin(ctx);
+        if ((rc = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "small reset")) ||
+            (rc = ctx_hip(ctx, launch_small(a, true, fp32, grid, ctx->stream), "small")))
+            return rc;
+        ctx->timer.end(ctx);
+        if (st) st->n_launches += 2;
+        if ((rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel"))) return rc;
+    }
+    return BCX_OK;
+}
+
+void small_release(bcx_ctx *ctx) {
+    if (!ctx->small) return;
+    SmallState &ss = *ctx->small;
+    for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm})
+        b->release(ctx);
+    delete ctx->small;
+    ctx->small = nullptr;
 }
 
 int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o, int32_t *level,

@@ -71,7 +71,8 @@ int make_plan(const HostComp &hc, const int32_t *subset, int64_t n_sub, bool use
 
 }  // namespace bcx
 
-struct BatchedState;  // batched path buffers (bcx_batched_host.cpp)
+struct BatchedState;  // batched path buffers (bcx_batched_host.hip)
+struct SmallState;    // small-component kernel buffers (bcx_batched_host.hip)
 
 struct bcx_ctx {
     int device = 0;
@@ -87,6 +88,7 @@ struct bcx_ctx {
     bcx::DBuf counters, bc_acc, plain_scratch, rows_buf;
     bcx::DBuf w_src, w_aux, w_mode, w_slots;
     BatchedState *batched = nullptr;
+    SmallState *small = nullptr;
     std::vector<bcx::DBuf *> all_bufs() {
         return {&counters, &bc_acc, &plain_scratch, &rows_buf, &w_src, &w_aux, &w_mode, &w_slots};
     }
@@ -108,7 +110,15 @@ int64_t row_capacity(bcx_ctx *ctx, int32_t n, const bcx_opts *o);
 int plain_component(bcx_ctx *ctx, const bcx::DComp &dc, const bcx::Plan &plan, const bcx_opts *o,
                     double *bc, bcx_stats *st);
 
-// batched path (bcx_batched_host.cpp)
+// batched path (bcx_batched_host.hip)
+struct SmallJob {
+    const bcx::DComp *dc;
+    std::vector<int32_t> srcs;
+};
+std::vector<int32_t> batched_sources(const bcx::DComp &dc, const bcx::Plan &plan);
+int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+                  bcx_stats *st);
+void small_release(bcx_ctx *ctx);
 int batched_component(bcx_ctx *ctx, bcx_dgraph *g, bcx::DComp &dc, const bcx::Plan &plan,
                       const bcx_opts *o, double *bc, bcx_stats *st);
 int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o,

@@ -19,6 +19,7 @@
 #include "bcx_device.h"
 #include "bcx_kernels.h"
 #include "bcx_runtime.h"
+#include "bcx_small.h"
 
 using namespace bcx;
 
@@ -453,6 +454,7 @@ void bcx_destroy(bcx_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (DBuf *b : ctx->all_bufs()) b->release(ctx);
     batched_release(ctx);
+    small_release(ctx);
     ctx->timer.destroy();
     ctx->timer_chain.destroy();
     (void)hipStreamDestroy(ctx->stream);
@@ -572,6 +574,7 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
             sub[it - g->comps.begin()].push_back((int32_t)(slot - it->offset));
         }
     }
+    std::vector<SmallJob> small;  // batched path: components for the one-launch small kernel
     for (size_t ci = 0; ci < g->comps.size(); ++ci) {
         DComp &dc = g->comps[ci];
         const int32_t *sp = o.src_subset ? sub[ci].data() : nullptr;
@@ -587,9 +590,11 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
                         (double)plan.n_chain * (8.0 * dc.arcs + 100.0 * dc.n);
         s->teps_work += (double)(plan.n_bfs + plan.n_chain) * (double)dc.arcs * 0.5;
         if (o.path == BCX_PATH_PLAIN) rc = plain_component(ctx, dc, plan, &o, bc + dc.offset, s);
+        else if (dc.n <= SMALL_N && !(o.flags & 1)) small.push_back({&dc, batched_sources(dc, plan)});
         else rc = batched_component(ctx, g, dc, plan, &o, bc + dc.offset, s);
         if (rc) return rc;
     }
+    if (!small.empty() && (rc = batched_small(ctx, small, &o, bc, s))) return rc;
     rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "run");
     if (rc) return rc;
     s->t_chain_s = ctx->timer_chain.collect();

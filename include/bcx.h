@@ -133,14 +133,19 @@ typedef struct {
     int32_t        path;          /* BCX_PATH_*                              */
     int32_t        precision;     /* BCX_PREC_*                              */
     int32_t        use_chains;    /* 1: chain vertices via endpoint rows     */
-    int32_t        batch_width;   /* batched: sources per tile (0 = auto)    */
+    int32_t        batch_width;   /* batched: concurrent tiles, 1..4 (0 = 4) */
     const int64_t *src_subset;    /* NULL = all sources; else slots          */
     int64_t        n_src;         /* slot = comp_offset[c] + local vertex    */
     int64_t        row_cap_bytes; /* endpoint-row slab cap (0 = auto)        */
     int32_t        sigma_f64_threshold_log2; /* test hook: treat sigma >=
                                      2^k as overflow (0 = 64, i.e. uint64)  */
-    int32_t        flags;         /* reserved, 0                              */
+    int32_t        flags;         /* BCX_FLAG_* bits                          */
 } bcx_opts;
+
+/* Batched path: run components of up to 2048 vertices through the
+ * multi-launch tile path instead of the one-launch small-component kernel
+ * (both compute the same; the flag exists for tests and measurements).     */
+#define BCX_FLAG_NO_SMALL_KERNEL 1
 
 typedef struct {
     double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */

@@ -112,3 +112,34 @@ def test_threshold_hook_and_subset(engine):
         assert mx < 1e-6 and zeros
     finally:
         g.free()
+
+
+@pytest.mark.parametrize("flags", [0, bcx.FLAG_NO_SMALL_KERNEL])
+def test_many_small_components(engine, flags):
+    """Workload 3 shape (a large BCC plus many small ones) at small scale:
+    the one-launch small-component kernel and the multi-launch tiles agree
+    with the oracle."""
+    n, uv = bcx.generate(3, 0, 0.002)
+    h = bcx.Host.from_edges(n, uv).decompose()
+    comps = h.components()
+    assert sum(c.n <= 2048 for c in comps) > 20
+    rp, col = oracle.csr_from_edges(n, uv)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    for prec, tol in ((bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)):
+        bc, st = engine.host_bc(h, path=B, precision=prec, flags=flags)
+        mx, _, zeros = relerr(bc, want)
+        print("flags", flags, "prec", prec, "maxrel", mx, st["n_launches"], flush=True)
+        assert mx < tol and zeros
+
+
+def test_small_kernel_fp64_sigma_rerun(engine):
+    """A small grid whose sigma passes a forced 2^6 threshold: the tiles are
+    listed and rerun with FP64 sigma inside the small-component kernel."""
+    w = 20
+    edges = [(y * w + x, y * w + x + 1) for y in range(w) for x in range(w - 1)]
+    edges += [(y * w + x, (y + 1) * w + x) for y in range(w - 1) for x in range(w)]
+    c = comps_of(w * w, edges)[0]
+    got, st = engine.bc([c], path=B, precision=bcx.PREC_FP32C, sigma_threshold_log2=6)
+    assert st["n_f64_sigma"] > 0
+    mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
+    assert mx < 1e-6 and zeros
