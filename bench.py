@@ -32,20 +32,11 @@ HBM_PEAK_GBS = 8000.0
 def bfs_rank(c):
     """BFS order from the highest-degree vertex, neighbours in CSR order
     (the tile order the batched driver uses)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import breadth_first_order
     deg = np.diff(c.row_ptr)
-    root = int(np.argmax(deg))
-    rank = np.full(c.n, -1, np.int64)
-    order = [root]
-    rank[root] = 0
-    rp, col = c.row_ptr, c.col
-    i = 0
-    while i < len(order):
-        v = order[i]
-        i += 1
-        for w in col[rp[v]:rp[v + 1]]:
-            if rank[w] < 0:
-                rank[w] = len(order)
-                order.append(int(w))
+    g = csr_matrix((np.ones(len(c.col), np.int8), c.col, c.row_ptr), shape=(c.n, c.n))
+    order = breadth_first_order(g, int(np.argmax(deg)), directed=True, return_predecessors=False)
     return np.asarray(order, np.int64)
 
 
@@ -86,6 +77,36 @@ def bytes_alg(comps, bfs, chn):
     return tot
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of `mib` MiB (hipMemcpy), for scaling
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_mib": mib, "copies": 3}))
+
+
+def measured_traffic(slices_used, src_per_step):
+    """HBM bytes per step from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (scaled per source when the slice differs)."""
+    p = os.path.join(ROOT, "profiles", "batched_config2_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        t = json.load(f)
+    per_src = t["hbm_bytes_per_step"] / t["sources_per_step"]
+    return per_src * src_per_step
+
+
 def timed_steps(g, slices, idx, warmup, **kw):
     for i in range(warmup):
         g.run(subset=slices[idx[i % len(idx)]][0], **kw)
@@ -113,7 +134,11 @@ def main():
     ap.add_argument("--cpu-sources", type=int, default=0, help="0 = 4 per host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only a device copy (profiling)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
@@ -194,6 +219,7 @@ def main():
                 "sources": int(len(sample))}
 
     achieved = balg_b / tk_b / 1e9 if tk_b > 0 else None
+    traffic = measured_traffic(args.slices, src_b / K)
     line = {
         "metric": "exact-BC MTEPS on 1 MI355X: batched FP32-comp vs plain, <=1e-6 rel err",
         "value": round(mteps_b, 3),
@@ -216,7 +242,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None},
+                     "traffic": round(traffic) if traffic else None,
+                     "traffic_unit": "HBM bytes per step (rocprofv3 FETCH_SIZE+WRITE_SIZE, scaled)",
+                     "algorithmic_bytes_per_step": round(balg_b / K),
+                     "kernel": "batched tile kernels (k_b_expand, k_b_pull, k_b_back) per step"},
         "cpu_baseline": cpu,
     }
     g.free()

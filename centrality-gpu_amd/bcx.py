@@ -265,6 +265,7 @@ class Host:
 
 
 FLAG_NO_SMALL_KERNEL = 1
+FLAG_NO_GRAPH = 2
 
 
 def make_opts(path=PATH_PLAIN, precision=PREC_FP64, use_chains=True, subset=None,

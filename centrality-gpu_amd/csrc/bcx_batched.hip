@@ -45,6 +45,7 @@ constexpr int NW = BB / 64;    // waves per block
 constexpr int UNR = 8;         // gathers in flight per lane in the sum loops
 constexpr int SMALL = 16;      // degree up to which one lane expands an entry
 constexpr int STAGE = 256, STAGE_FLUSH = 192;
+constexpr int EB = 16;         // list entries per wave block in the backward
 
 __device__ __forceinline__ int gwave() { return (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6); }
 __device__ __forceinline__ int nwaves() { return (int)((gridDim.x * (unsigned)blockDim.x) >> 6); }
@@ -241,17 +242,24 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
         const uint64_t bal = __ballot(N0 != 0);
         if (!bal) continue;
         const int k0 = cnt;
+        // CSR ranges of the chunk's entries, loaded lane-parallel up front
+        int r0 = 0, r1 = 0;
         if (N0) {
             const int slot = k0 + __popcll(bal & (me - 1));
             s_w[wv][slot] = w0;
             s_m[wv][slot] = N0;
+            r0 = b.rp[w0];
+            r1 = b.rp[w0 + 1];
         }
         cnt += __popcll(bal);
         __builtin_amdgcn_wave_barrier();
+        uint64_t rest = bal;
         for (int k = k0; k < cnt; ++k) {
-            const int w = s_w[wv][k];
-            const uint64_t N = s_m[wv][k];
-            const int a0 = b.rp[w], a1 = b.rp[w + 1];
+            const int src = __ffsll((long long)rest) - 1;
+            rest &= rest - 1;
+            const int w = __shfl(w0, src, 64);
+            const uint64_t N = shfl64(N0, src);
+            const int a0 = __shfl(r0, src, 64), a1 = __shfl(r1, src, 64);
             if (!(split && is_hub(b, a1 - a0))) {
                 uint64_t acc = 0;
                 double facc = 0.0;
@@ -382,13 +390,27 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     uint64_t *clr = mbuf(b, l + 2);
     for (int i = c0 + blockIdx.x * BB + threadIdx.x; i < c1; i += gridDim.x * BB) clr[b.Q[i]] = 0;
     const int lane = threadIdx.x & 63;
-    for (int e = q0 + gwave(); e < q1; e += nwaves()) {
-        const int v = b.Q[e];
-        const uint64_t M = b.Qm[e];
-        if (lane == 0) own[v] = M;
-        const int a0 = b.rp[v], a1 = b.rp[v + 1];
-        if (is_hub(b, a1 - a0) && l < b.hub_stride) continue;  // hub slices below
-        finish_vertex<F64, TT>(b, v, M, rho_sum<TT>(b, succ, M, a0, a1));
+    // blocks of EB entries per wave: vertex, mask and CSR range are loaded
+    // lane-parallel, then the entries are walked one by one
+    for (int base = q0 + gwave() * EB; base < q1; base += nwaves() * EB) {
+        const int e = base + lane;
+        int v0 = 0, r0 = 0, r1 = 0;
+        uint64_t M0 = 0;
+        if (lane < EB && e < q1) {
+            v0 = b.Q[e];
+            M0 = b.Qm[e];
+            own[v0] = M0;
+            r0 = b.rp[v0];
+            r1 = b.rp[v0 + 1];
+        }
+        const int cnt = min(EB, q1 - base);
+        for (int k = 0; k < cnt; ++k) {
+            const int a0 = __shfl(r0, k, 64), a1 = __shfl(r1, k, 64);
+            if (is_hub(b, a1 - a0) && l < b.hub_stride) continue;  // hub slices below
+            const int v = __shfl(v0, k, 64);
+            const uint64_t M = shfl64(M0, k);
+            finish_vertex<F64, TT>(b, v, M, rho_sum<TT>(b, succ, M, a0, a1));
+        }
     }
     // hub slices (levels with recorded hub masks): partial rho into FP64
     // accumulators; the last slice of a hub to finish (counter) completes it

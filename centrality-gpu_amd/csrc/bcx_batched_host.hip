@@ -395,7 +395,46 @@ int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, c
 struct SmallState {
     DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm;
     int grid = 0;
+    // captured launch (counter reset kernel + persistent kernel), replayed
+    // while its arguments stay the same
+    hipGraphExec_t exec = nullptr;
+    SmallArgs key{};
+    int key_grid = 0;
+    bool key_f64 = false, key_fp32 = false;
 };
+
+// Reset + persistent kernel, replayed from a captured graph when possible.
+static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool f64, bool fp32, int grid,
+                        bool use_graph) {
+    if (!use_graph) {
+        int rc = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "small reset");
+        return rc ? rc : ctx_hip(ctx, launch_small(a, f64, fp32, grid, ctx->stream), "small");
+    }
+    const bool same = ss.exec && !std::memcmp(&ss.key, &a, sizeof(a)) && ss.key_grid == grid &&
+                      ss.key_f64 == f64 && ss.key_fp32 == fp32;
+    if (!same) {
+        if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
+        ss.exec = nullptr;
+        hipGraph_t graph = nullptr;
+        int rc = ctx_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "capture");
+        if (rc) return rc;
+        hipError_t e1 = launch_small_reset(a.counter, ctx->stream);
+        hipError_t e2 = launch_small(a, f64, fp32, grid, ctx->stream);
+        hipError_t e3 = hipStreamEndCapture(ctx->stream, &graph);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return ctx_hip(ctx, e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "capture");
+        }
+        hipError_t e4 = hipGraphInstantiate(&ss.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e4 != hipSuccess) { ss.exec = nullptr; return ctx_hip(ctx, e4, "graph instantiate"); }
+        std::memcpy(&ss.key, &a, sizeof(a));
+        ss.key_grid = grid;
+        ss.key_f64 = f64;
+        ss.key_fp32 = fp32;
+    }
+    return ctx_hip(ctx, hipGraphLaunch(ss.exec, ctx->stream), "graph launch");
+}
 
 int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
                   bcx_stats *st) {
@@ -461,10 +500,9 @@ int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opt
     a.limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
                   ? (1ull << o->sigma_f64_threshold_log2) : 0;
     const bool fp32 = o->precision == BCX_PREC_FP32C;
+    const bool use_graph = !(o->flags & BCX_FLAG_NO_GRAPH);
     ctx->timer.begin(ctx);
-    if ((rc = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "small reset")) ||
-        (rc = ctx_hip(ctx, launch_small(a, false, fp32, grid, ctx->stream), "small")))
-        return rc;
+    if ((rc = small_launch(ctx, ss, a, false, fp32, grid, use_graph))) return rc;
     ctx->timer.end(ctx);
     if (st) st->n_launches += 2;
     int32_t h[2];
@@ -487,9 +525,7 @@ int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opt
         a.limit = 0;
         if (st) for (auto &t : again) st->n_f64_sigma += t.ns;
         ctx->timer.begin(ctx);
-        if ((rc = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "small reset")) ||
-            (rc = ctx_hip(ctx, launch_small(a, true, fp32, grid, ctx->stream), "small")))
-            return rc;
+        if ((rc = small_launch(ctx, ss, a, true, fp32, grid, false))) return rc;
         ctx->timer.end(ctx);
         if (st) st->n_launches += 2;
         if ((rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel"))) return rc;
@@ -500,6 +536,7 @@ int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opt
 void small_release(bcx_ctx *ctx) {
     if (!ctx->small) return;
     SmallState &ss = *ctx->small;
+    if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
     for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm})
         b->release(ctx);
     delete ctx->small;

@@ -146,6 +146,9 @@ typedef struct {
  * multi-launch tile path instead of the one-launch small-component kernel
  * (both compute the same; the flag exists for tests and measurements).     */
 #define BCX_FLAG_NO_SMALL_KERNEL 1
+/* Batched path: launch the small-component kernel eagerly instead of
+ * replaying its captured hipGraph.                                         */
+#define BCX_FLAG_NO_GRAPH        2
 
 typedef struct {
     double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */

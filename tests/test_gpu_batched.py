@@ -143,3 +143,20 @@ def test_small_kernel_fp64_sigma_rerun(engine):
     assert st["n_f64_sigma"] > 0
     mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
     assert mx < 1e-6 and zeros
+
+
+def test_small_kernel_graph_replays(engine):
+    """The captured small-component launch replayed several times, with eager
+    work and synchronisations in between, matches the eager launch every time."""
+    n, uv = bcx.generate(3, 0, 0.001)
+    comps = [c for c in bcx.Host.from_edges(n, uv).decompose().components() if c.n <= 2048]
+    g = engine.upload(comps)
+    try:
+        eager, _ = g.run(path=B, precision=bcx.PREC_FP64, flags=bcx.FLAG_NO_GRAPH)
+        for rep in range(4):
+            got, _ = g.run(path=B, precision=bcx.PREC_FP64)
+            np.testing.assert_allclose(got, eager, rtol=1e-12, atol=0, err_msg=f"replay {rep}")
+            other, _ = g.run(path=bcx.PATH_PLAIN, precision=bcx.PREC_FP64)  # eager work between replays
+            np.testing.assert_allclose(other, eager, rtol=1e-9, atol=0)
+    finally:
+        g.free()

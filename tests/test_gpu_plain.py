@@ -181,3 +181,14 @@ def test_cli_sample_inputs(goldens, tmp_path):
         mx, _, zeros = relerr(bc, goldens[f"sample_test{k}"]["bc"])
         assert mx < BC_TOL and zeros
         assert (tmp_path / "PostProcessOp.txt").exists()
+
+
+def test_cli_batched_fp32(goldens, tmp_path):
+    exe = os.path.join(bcx.HERE, "betc")
+    src = os.path.join(bcx.HERE, "..", "tests", "golden", "test4.txt")
+    out = tmp_path / "bc.txt"
+    r = subprocess.run([exe, src, "--batched", "--fp32", "--bc-out-exact", str(out)], capture_output=True,
+                       text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    mx, _, zeros = relerr(np.loadtxt(out)[:, 1], goldens["sample_test4"]["bc"])
+    assert mx < 1e-6 and zeros
