@@ -36,7 +36,7 @@ struct BatchDev {
     int32_t *hub_cnt;                 // [n_hub] slices finished this level
     const int32_t *srcs;              // [64] source per lane
     const double *ws;                 // [64] source weight r(s)+1 per lane
-    double *bc;                       // component slice of the BC accumulator
+    double *bc;                       // the slot's BC accumulator (folded per component)
     uint64_t limit;                   // u64 sigma threshold (0 = 2^64)
 };
 
@@ -47,5 +47,6 @@ hipError_t launch_b_forward(const BatchDev &b, bool f64, int l, int grid, hipStr
 hipError_t launch_b_back(const BatchDev &b, bool f64, bool fp32, int l, int grid, hipStream_t st);
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st);
 hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st);
+hipError_t launch_b_fold(double *dst, double *src, int n, int grid, hipStream_t st);
 
 }  // namespace bcx

@@ -45,7 +45,8 @@ constexpr int NW = BB / 64;    // waves per block
 constexpr int UNR = 8;         // gathers in flight per lane in the sum loops
 constexpr int SMALL = 16;      // degree up to which one lane expands an entry
 constexpr int STAGE = 256, STAGE_FLUSH = 192;
-constexpr int EB = 16;         // list entries per wave block in the backward
+constexpr int FEB = 8;         // list entries per wave block in the backward
+constexpr int WIN = 128;       // arcs per window of a block
 
 __device__ __forceinline__ int gwave() { return (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6); }
 __device__ __forceinline__ int nwaves() { return (int)((gridDim.x * (unsigned)blockDim.x) >> 6); }
@@ -315,8 +316,13 @@ struct SumD {
     __device__ __forceinline__ double value() const { return s; }
 };
 
-template <typename TT>
-__device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *succ, uint64_t M, int a0, int a1) {
+// Sum over successors w (lanes in succ[w] & M) of the dependency term.
+// rho form (DF false): t_w, the sum is rho_v.  Successor form (DF true, FP64
+// sigma with f32 terms, where 1/sigma underflows f32): u_w = r(w)+1+delta_w
+// scaled by the FP64 ratio sigma_v/sigma_w rounded to f32, the sum is delta_v.
+template <typename TT, bool DF = false>
+__device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *succ, uint64_t M, int a0, int a1,
+                                          double sv = 0.0) {
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
     const TT *tt = (const TT *)b.tt;
     const int lane = threadIdx.x & 63;
@@ -333,11 +339,13 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
         uint64_t bal = __ballot(x != 0);
         while (bal) {
             TT t[UNR];
+            uint64_t sw[UNR];
             bool use[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 use[u] = false;
                 t[u] = 0;
+                sw[u] = 0;
                 if (bal) {
                     const int src = __ffsll((long long)bal) - 1;
                     bal &= bal - 1;
@@ -346,38 +354,51 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
                     if (xx & me) {
                         use[u] = true;
                         t[u] = tt[(int64_t)ww * 64 + lane];
+                        if (DF) sw[u] = b.sig[(int64_t)ww * 64 + lane];
                     }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < UNR; ++u)
-                if (use[u]) acc.add(t[u]);
+            for (int u = 0; u < UNR; ++u) {
+                if (!use[u]) continue;
+                if (DF) acc.add((TT)(sv / u2d(sw[u])) * t[u]);
+                else acc.add(t[u]);
+            }
         }
     }
     return acc.value();
 }
 
 // rho complete for (v, lanes M): delta, BC contribution and the term t_v.
+// `acc` is rho_v (rho form) or delta_v (successor form, see rho_sum).
 template <bool F64, typename TT>
-__device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t M, double rho) {
+__device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t M, double acc) {
+    constexpr bool DF = F64 && sizeof(TT) == 4;
     const int lane = threadIdx.x & 63;
     double contrib = 0.0;
     if (M & (1ull << lane)) {
         const uint64_t sx = b.sig[(int64_t)v * 64 + lane];
         const double sv = F64 ? u2d(sx) : (double)sx;
-        const double delta = sv * rho;
+        const double delta = DF ? acc : sv * acc;
         contrib = b.ws[lane] * delta;
-        ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + rho);
+        ((TT *)b.tt)[(int64_t)v * 64 + lane] =
+            DF ? (TT)((double)b.reach[v] + 1.0 + delta) : (TT)(((double)b.reach[v] + 1.0) / sv + acc);
         if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
     }
     contrib = wave_sum(contrib);
-    if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&b.bc[v], contrib);
+    // b.bc is the slot's own accumulator and v is finished once per level
+    // kernel, so a plain read-modify-write is race free
+    if (lane == 0 && contrib != 0.0) b.bc[v] += contrib;
 }
 
 // Level l <= depth: entries Q_l; Q_{l+2} is cleared from mb[(l+2)%3] and Q_l
 // scattered into mb[l%3] for level l-1.
 template <bool F64, typename TT>
 __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
+    __shared__ int t_w[NW][WIN];
+    __shared__ uint64_t t_x[NW][WIN];
+    __shared__ int t_k[NW][WIN];
+    __shared__ double r_rho[NW][FEB][64];
     // the host launches the backward speculatively behind the forward: a
     // level beyond the depth, an unfinished forward (depth -1), a sigma
     // overflow (the tile is redone in FP64) or full lists all mean no work
@@ -389,27 +410,147 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     const uint64_t *succ = mbuf(b, l + 1);
     uint64_t *clr = mbuf(b, l + 2);
     for (int i = c0 + blockIdx.x * BB + threadIdx.x; i < c1; i += gridDim.x * BB) clr[b.Q[i]] = 0;
-    const int lane = threadIdx.x & 63;
-    // blocks of EB entries per wave: vertex, mask and CSR range are loaded
-    // lane-parallel, then the entries are walked one by one
-    for (int base = q0 + gwave() * EB; base < q1; base += nwaves() * EB) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t me = 1ull << lane;
+    using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
+    constexpr bool DF = F64 && sizeof(TT) == 4;  // successor form (see rho_sum)
+    const TT *tt = (const TT *)b.tt;
+    // Blocks of FEB list entries per wave.  The arcs of the block are walked
+    // as one flat range (lane per arc) in windows of WIN: successor masks
+    // are gathered for 64 arcs at once and the (entry, successor, lanes)
+    // triples that matter are compacted in LDS; the t rows of those triples
+    // are then gathered UNR at a time, accumulated per entry in arc order.
+    for (int base = q0 + gwave() * FEB; base < q1; base += nwaves() * FEB) {
         const int e = base + lane;
         int v0 = 0, r0 = 0, r1 = 0;
         uint64_t M0 = 0;
-        if (lane < EB && e < q1) {
+        if (lane < FEB && e < q1) {
             v0 = b.Q[e];
             M0 = b.Qm[e];
             own[v0] = M0;
             r0 = b.rp[v0];
             r1 = b.rp[v0 + 1];
         }
-        const int cnt = min(EB, q1 - base);
-        for (int k = 0; k < cnt; ++k) {
-            const int a0 = __shfl(r0, k, 64), a1 = __shfl(r1, k, 64);
-            if (is_hub(b, a1 - a0) && l < b.hub_stride) continue;  // hub slices below
-            const int v = __shfl(v0, k, 64);
-            const uint64_t M = shfl64(M0, k);
-            finish_vertex<F64, TT>(b, v, M, rho_sum<TT>(b, succ, M, a0, a1));
+        const int cnt = min(FEB, q1 - base);
+        const bool mine = lane < cnt && !(is_hub(b, r1 - r0) && l < b.hub_stride);  // hubs: slices
+        const int d = mine ? r1 - r0 : 0;
+        int inc = d;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        const int pre = inc - d;
+        const int T = __shfl(inc, 63, 64);
+        int P[FEB], A[FEB], V[FEB];
+        uint64_t MM[FEB];
+        bool OWN[FEB];
+#pragma unroll
+        for (int j = 0; j < FEB; ++j) {
+            P[j] = j < cnt ? __shfl(pre, j, 64) : INT32_MAX;
+            A[j] = __shfl(r0, j, 64);
+            V[j] = __shfl(v0, j, 64);
+            MM[j] = shfl64(M0, j);
+            OWN[j] = __shfl((int)mine, j, 64) != 0;
+            r_rho[wv][j][lane] = 0.0;
+        }
+        Acc acc;
+        int kcur = -1;
+        double svcur = 0.0;  // successor form: sigma of entry kcur on this lane
+        for (int win = 0; win < T; win += WIN) {
+            const int wend = min(T, win + WIN);
+            int nrel = 0;
+            for (int c = win; c < wend; c += 64) {
+                const int g = c + lane;
+                int k = 0;
+#pragma unroll
+                for (int j = 1; j < FEB; ++j) k += (P[j] <= g);
+                int a0k = A[0], pk = P[0];
+                uint64_t mk = MM[0];
+#pragma unroll
+                for (int j = 1; j < FEB; ++j)
+                    if (k == j) { a0k = A[j]; pk = P[j]; mk = MM[j]; }
+                int w = 0;
+                uint64_t x = 0;
+                if (g < wend) {
+                    w = b.col[a0k + g - pk];
+                    x = succ[w] & mk;
+                }
+                const uint64_t bal = __ballot(x != 0);
+                if (x) {
+                    const int pos = nrel + __popcll(bal & (me - 1));
+                    t_w[wv][pos] = w;
+                    t_x[wv][pos] = x;
+                    t_k[wv][pos] = k;
+                }
+                nrel += __popcll(bal);
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int r = 0; r < nrel; r += UNR) {
+                TT t[UNR];
+                uint64_t sw[UNR];
+                int kk[UNR];
+                bool use[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    const int rr = r + u;
+                    use[u] = false;
+                    kk[u] = -1;
+                    t[u] = 0;
+                    sw[u] = 0;
+                    if (rr < nrel) {
+                        kk[u] = t_k[wv][rr];
+                        if (t_x[wv][rr] & me) {
+                            use[u] = true;
+                            t[u] = tt[(int64_t)t_w[wv][rr] * 64 + lane];
+                            if (DF) sw[u] = b.sig[(int64_t)t_w[wv][rr] * 64 + lane];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    if (kk[u] < 0) continue;
+                    if (kk[u] != kcur) {
+                        if (kcur >= 0) r_rho[wv][kcur][lane] = acc.value();
+                        acc = Acc();
+                        kcur = kk[u];
+                        if (DF) {
+                            int vk = V[0];
+#pragma unroll
+                            for (int j = 1; j < FEB; ++j)
+                                if (kcur == j) vk = V[j];
+                            svcur = u2d(b.sig[(int64_t)vk * 64 + lane]);  // used only where lane is in the entry's mask
+                        }
+                    }
+                    if (use[u]) {
+                        if (DF) acc.add((TT)(svcur / u2d(sw[u])) * t[u]);
+                        else acc.add(t[u]);
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (kcur >= 0) r_rho[wv][kcur][lane] = acc.value();
+        // finish every entry of the block (hubs excepted): sigma rows first
+        uint64_t SX[FEB];
+#pragma unroll
+        for (int j = 0; j < FEB; ++j) SX[j] = (OWN[j] && (MM[j] & me)) ? b.sig[(int64_t)V[j] * 64 + lane] : 0;
+#pragma unroll
+        for (int j = 0; j < FEB; ++j) {
+            if (!OWN[j]) continue;
+            const int v = V[j];
+            double contrib = 0.0;
+            if (MM[j] & me) {
+                const double sv = F64 ? u2d(SX[j]) : (double)SX[j];
+                const double a = r_rho[wv][j][lane];  // rho_v, or delta_v in the successor form
+                const double delta = DF ? a : sv * a;
+                contrib = b.ws[lane] * delta;
+                ((TT *)b.tt)[(int64_t)v * 64 + lane] =
+                    DF ? (TT)((double)b.reach[v] + 1.0 + delta) : (TT)(((double)b.reach[v] + 1.0) / sv + a);
+                if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
+            }
+            contrib = wave_sum(contrib);
+            if (lane == 0 && contrib != 0.0) b.bc[v] += contrib;
         }
     }
     // hub slices (levels with recorded hub masks): partial rho into FP64
@@ -420,7 +561,8 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         const uint64_t M = b.hub_mask[(int64_t)h * b.hub_stride + l];
         if (!M) continue;
         const int v = b.hub_v[h];
-        const double part = rho_sum<TT>(b, succ, M, b.sl_j0[s], b.sl_j1[s]);
+        const double svh = DF && (M & (1ull << lane)) ? u2d(b.sig[(int64_t)v * 64 + lane]) : 0.0;
+        const double part = rho_sum<TT, DF>(b, succ, M, b.sl_j0[s], b.sl_j1[s], svh);
         if ((M & (1ull << lane)) && part != 0.0) unsafeAtomicAdd(&b.hub_acc[h * 64 + lane], part);
         __threadfence();
         int done = 0;
@@ -454,6 +596,16 @@ __global__ __launch_bounds__(BB) void k_b_clear_all(BatchDev b, int end) {
     }
 }
 
+// dst[v] += src[v]: folds a slot's BC accumulator into the component slice
+// and clears it for the next component.
+__global__ __launch_bounds__(BB) void k_b_fold(double *dst, double *src, int n) {
+    for (int v = blockIdx.x * BB + threadIdx.x; v < n; v += gridDim.x * BB) {
+        const double x = src[v];
+        if (x != 0.0) dst[v] += x;
+        src[v] = 0.0;
+    }
+}
+
 // ------------------------------------------------------------ launchers ---
 hipError_t launch_b_init(const BatchDev &b, int ns, bool f64, int cap, hipStream_t st) {
     int g = (int)std::min<int64_t>(((int64_t)b.n + BB - 1) / BB, cap);
@@ -484,6 +636,11 @@ hipError_t launch_b_back(const BatchDev &b, bool f64, bool fp32, int l, int grid
 
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_b_finish, dim3(grid), dim3(BB), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_b_fold(double *dst, double *src, int n, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_b_fold, dim3(grid), dim3(BB), 0, st, dst, src, n);
     return hipGetLastError();
 }
 

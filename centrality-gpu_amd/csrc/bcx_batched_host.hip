@@ -26,7 +26,7 @@ constexpr int HUB_DEG = 256, HUB_SLICE = 256, HUB_LEVELS = 1024;
 constexpr int MAX_SLOTS = 4;
 
 struct Slot {
-    DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, hub_mask, hub_acc, hub_cnt, lvl, dbg;
+    DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, hub_mask, hub_acc, hub_cnt, lvl, dbg, bcacc;
     int32_t n_cap = 0, qcap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -58,9 +58,12 @@ static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
         if ((rc = s.vis.ensure(ctx, 8 * N, "batched vis")) || (rc = s.mb.ensure(ctx, 24 * N, "batched masks")) ||
             (rc = s.sig.ensure(ctx, 8 * 64 * N, "batched sigma")) ||
             (rc = s.tt.ensure(ctx, 8 * 64 * N, "batched dependency terms")) ||
-            (rc = s.qoff.ensure(ctx, 4 * (N + 8), "batched level offsets")))
+            (rc = s.qoff.ensure(ctx, 4 * (N + 8), "batched level offsets")) ||
+            (rc = s.bcacc.ensure(ctx, 8 * N, "batched BC accumulator")))
             return rc;
-        if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, s.stream), "memset"))) return rc;
+        if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, s.stream), "memset")) ||
+            (rc = ctx_hip(ctx, hipMemsetAsync(s.bcacc.p, 0, s.bcacc.bytes, s.stream), "memset")))
+            return rc;
         s.n_cap = n;
     }
     if (s.qcap < 8 * n + 4096) {
@@ -132,7 +135,8 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uin
     b.hub_cnt = (int32_t *)s.hub_cnt.p;
     b.srcs = (const int32_t *)bs.srcs.p + (size_t)s.tile * BATCH_W;
     b.ws = (const double *)bs.ws.p + (size_t)s.tile * BATCH_W;
-    b.bc = bc;
+    b.bc = (double *)s.bcacc.p;
+    (void)bc;
     b.limit = limit;
     return b;
 }
@@ -265,6 +269,9 @@ static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob
         (void)hipEventRecord(bs.slots[i].done, bs.slots[i].stream);
         (void)hipStreamWaitEvent(ctx->stream, bs.slots[i].done, 0);
     }
+    // each slot accumulated its tiles' BC privately: fold into the component
+    for (int i = 0; i < ns_slots && !rc; ++i)
+        rc = ctx_hip(ctx, launch_b_fold(j.bc, (double *)bs.slots[i].bcacc.p, dc.n, j.grid, ctx->stream), "fold");
     ctx->timer.end(ctx);
     for (int i = 0; i < ns_slots; ++i) (void)hipStreamSynchronize(bs.slots[i].stream);
     (void)hipEventDestroy(e0);
@@ -594,7 +601,7 @@ void batched_release(bcx_ctx *ctx) {
     for (Slot &s : bs.slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.qoff, &s.hub_mask, &s.hub_acc,
-                        &s.hub_cnt, &s.lvl, &s.dbg})
+                        &s.hub_cnt, &s.lvl, &s.dbg, &s.bcacc})
             b->release(ctx);
         if (s.pinned) (void)hipHostFree(s.pinned);
         if (s.done) (void)hipEventDestroy(s.done);

@@ -160,3 +160,23 @@ def test_small_kernel_graph_replays(engine):
             np.testing.assert_allclose(other, eager, rtol=1e-9, atol=0)
     finally:
         g.free()
+
+
+@pytest.mark.parametrize("flags", [0, bcx.FLAG_NO_SMALL_KERNEL])
+def test_fp32c_sigma_beyond_f32_range(engine, flags):
+    """Layers of 3 vertices, consecutive layers completely joined: sigma
+    reaches 3^99 (> 1e47), beyond f32 range, so 1/sigma underflows f32.  The
+    FP64-sigma tiles of the FP32-compensated mode keep BC within 1e-6, both in
+    the small-component kernel and in the multi-launch tiles."""
+    L, k = 100, 3
+    edges = [(i * k + a, (i + 1) * k + b) for i in range(L - 1) for a in range(k) for b in range(k)]
+    c = comps_of(L * k, edges, chains=False)[0]
+    assert c.n == L * k
+    got, st = engine.bc([c], path=B, precision=bcx.PREC_FP32C, flags=flags)
+    assert st["n_f64_sigma"] > 0
+    want = oracle.component_bc(c.row_ptr, c.col, c.reach)
+    mx, _, zeros = relerr(got[0], want)
+    print("flags", flags, "maxrel", mx, flush=True)
+    assert mx < 1e-6 and zeros
+    for s in (0, k * L // 2):
+        check_source(engine, c, s, bcx.PREC_FP32C)
