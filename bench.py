@@ -135,6 +135,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only a device copy (profiling)")
+    ap.add_argument("--slots", type=int, default=0, help="concurrent tiles of the batched path (0 = library default)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -165,7 +166,7 @@ def main():
     K = args.steps
     idx = list(range(K))
     t_b, tk_b, st_b = timed_steps(g, slices, idx, args.warmup, path=bcx.PATH_BATCHED,
-                                  precision=bcx.PREC_FP32C, use_chains=True)
+                                  precision=bcx.PREC_FP32C, use_chains=True, batch_width=args.slots)
     src_b = sum(len(slices[i][0]) for i in idx)
     work_b = W * src_b / total_src
     mteps_b = work_b / t_b / 1e6

@@ -14,7 +14,7 @@ struct BatchDev {
     int32_t n;
     const int32_t *rp, *col, *reach;
     uint64_t *vis, *mb0, *mb1, *mb2;  // lanes reached; rotating level masks
-    uint64_t *sig;                    // [n*64] u64 or f64 bits
+    void *sig;                        // [n*64] u32, u64 or f64 bits (the tile's sigma mode)
     void *tt;                         // [n*64] float or double
     uint16_t *lvl;                    // optional [n*64] levels (probe)
     double *dbg_delta;                // optional [n*64] dependencies (probe)
@@ -37,14 +37,19 @@ struct BatchDev {
     const int32_t *srcs;              // [64] source per lane
     const double *ws;                 // [64] source weight r(s)+1 per lane
     double *bc;                       // the slot's BC accumulator (folded per component)
-    uint64_t limit;                   // u64 sigma threshold (0 = 2^64)
+    uint64_t limit;                   // integer sigma threshold (0 = the word's range)
 };
+
+// Sigma word of a tile: 32-bit integers, 64-bit integers, or FP64 summed in
+// CSR order.  A tile starts in one mode and moves up one mode when a count
+// leaves the word's range (or passes `limit`), rerunning its forward.
+constexpr int SIG_U32 = 0, SIG_U64 = 1, SIG_F64 = 2;
 
 // Every level kernel reads its list ranges from b.qoff / b.ctr, so a whole
 // tile is a fixed launch sequence with no host round trip per level.
-hipError_t launch_b_init(const BatchDev &b, int ns, bool f64, int cap, hipStream_t st);
-hipError_t launch_b_forward(const BatchDev &b, bool f64, int l, int grid, hipStream_t st);
-hipError_t launch_b_back(const BatchDev &b, bool f64, bool fp32, int l, int grid, hipStream_t st);
+hipError_t launch_b_init(const BatchDev &b, int ns, int sm, int cap, hipStream_t st);
+hipError_t launch_b_forward(const BatchDev &b, int sm, int l, int grid, hipStream_t st);
+hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st);
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st);
 hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st);
 hipError_t launch_b_fold(double *dst, double *src, int n, int grid, hipStream_t st);

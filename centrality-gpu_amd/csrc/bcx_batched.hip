@@ -5,7 +5,8 @@
 //   mb[l % 3][v]  u64  lanes for which v is at level l (three rotating dense
 //                      mask arrays; only entries of the level lists are ever
 //                      non-zero, cleared incrementally)
-//   sig[v][64]    u64 (exact) or f64 (overflow mode) path counts per lane
+//   sig[v][64]    u32 or u64 (exact) or f64 (overflow mode) path counts per
+//                 lane; the narrowest word that holds the tile's counts
 //   tt[v][64]     f32 or f64 backward term t_v = (r(v)+1)/sigma_v + rho_v
 //   Q / Qm        per-level lists of (vertex, lane mask), levels concatenated,
 //                 level l = Q[qoff[l], qoff[l+1])
@@ -66,21 +67,44 @@ __device__ __forceinline__ int ldi(const int32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// sigma word i of the tile in mode SM, widened to 64 bits (f64 as its bits)
+template <int SM>
+__device__ __forceinline__ uint64_t sg(const BatchDev &b, int64_t i) {
+    if (SM == SIG_U32) return ((const uint32_t *)b.sig)[i];
+    return ((const uint64_t *)b.sig)[i];
+}
+template <int SM>
+__device__ __forceinline__ void sg_put(const BatchDev &b, int64_t i, uint64_t x) {
+    if (SM == SIG_U32) ((uint32_t *)b.sig)[i] = (uint32_t)x;
+    else ((uint64_t *)b.sig)[i] = x;
+}
+template <int SM>
+__device__ __forceinline__ double sg_val(uint64_t x) {
+    return SM == SIG_F64 ? u2d(x) : (double)x;
+}
+// largest count the mode's word holds
+template <int SM>
+__device__ __forceinline__ uint64_t sg_max() {
+    return SM == SIG_U32 ? 0xffffffffull : ~0ull;
+}
+
 // ---------------------------------------------------------------- init ---
-__global__ __launch_bounds__(BB) void k_b_clear_vis(BatchDev b) {
+__global__ __launch_bounds__(BB) void k_b_clear_vis(BatchDev b, int sm) {
     const int64_t stride = (int64_t)gridDim.x * BB;
     const int64_t t0 = blockIdx.x * (int64_t)BB + threadIdx.x;
     for (int64_t v = t0; v < b.n; v += stride) b.vis[v] = 0;
     // hub sigma rows and backward accumulators start at zero every tile: each
     // (hub, lane) is summed into exactly once
     for (int64_t i = t0; i < (int64_t)b.n_hub * 64; i += stride) {
-        b.sig[(int64_t)b.hub_v[i >> 6] * 64 + (i & 63)] = 0;
+        const int64_t k = (int64_t)b.hub_v[i >> 6] * 64 + (i & 63);
+        if (sm == SIG_U32) sg_put<SIG_U32>(b, k, 0);
+        else sg_put<SIG_U64>(b, k, 0);
         b.hub_acc[i] = 0.0;
     }
     for (int64_t i = t0; i < b.n_hub; i += stride) b.hub_cnt[i] = 0;
 }
 
-template <bool F64>
+template <int SM>
 __global__ void k_b_seed(BatchDev b, int ns) {
     const int k = threadIdx.x;
     if (k < ns) {
@@ -88,7 +112,7 @@ __global__ void k_b_seed(BatchDev b, int ns) {
         const uint64_t bit = 1ull << k;
         b.vis[s] = bit;
         b.mb0[s] = bit;
-        b.sig[(int64_t)s * 64 + k] = F64 ? d2u(1.0) : 1ull;
+        sg_put<SM>(b, (int64_t)s * 64 + k, SM == SIG_F64 ? d2u(1.0) : 1ull);
         b.Q[k] = s;
         b.Qm[k] = bit;
         if (b.lvl) b.lvl[(int64_t)s * 64 + k] = 0;
@@ -159,7 +183,7 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
 }
 
 // --------------------------------------------------------------- pull ---
-template <bool F64>
+template <int SM>
 __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, uint64_t N, int a0, int a1,
                                           uint64_t &acc, double &facc, bool &ovf) {
     const int lane = threadIdx.x & 63;
@@ -187,14 +211,14 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
                     const uint64_t ff = shfl64(f, src);
                     if (ff & me) {
                         use[u] = true;
-                        x[u] = b.sig[(int64_t)vv * 64 + lane];
+                        x[u] = sg<SM>(b, (int64_t)vv * 64 + lane);
                     }
                 }
             }
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 if (!use[u]) continue;
-                if (F64) {
+                if (SM == SIG_F64) {
                     facc += u2d(x[u]);
                 } else {
                     const uint64_t s = acc + x[u];
@@ -226,8 +250,9 @@ __device__ __forceinline__ void stage_flush(const BatchDev &b, int *sw, uint64_t
 }
 
 // Builds Q_{l+1} from the non-zero entries of mb[(l+1)%3] and their sigma.
-template <bool F64>
+template <int SM>
 __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
+    constexpr bool F64 = SM == SIG_F64;
     __shared__ int s_w[NW][STAGE];
     __shared__ uint64_t s_m[NW][STAGE];
     if (ldi(&b.ctr[3]) >= 0) return;
@@ -265,10 +290,10 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
                 uint64_t acc = 0;
                 double facc = 0.0;
                 bool ovf = false;
-                sigma_sum<F64>(b, fr, N, a0, a1, acc, facc, ovf);
+                sigma_sum<SM>(b, fr, N, a0, a1, acc, facc, ovf);
                 if (N & me) {
-                    b.sig[(int64_t)w * 64 + lane] = F64 ? d2u(facc) : acc;
-                    if (!F64 && (ovf || (b.limit && acc >= b.limit))) b.ctr[1] = 1;
+                    sg_put<SM>(b, (int64_t)w * 64 + lane, F64 ? d2u(facc) : acc);
+                    if (!F64 && (ovf || acc > sg_max<SM>() || (b.limit && acc >= b.limit))) b.ctr[1] = 1;
                 }
             }
             if ((N & me) && b.lvl) b.lvl[(int64_t)w * 64 + lane] = (uint16_t)(l + 1);
@@ -289,12 +314,20 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
         uint64_t acc = 0;
         double facc = 0.0;
         bool ovf = false;
-        sigma_sum<false>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf);
+        constexpr int SI = F64 ? SIG_U64 : SM;  // (not reached in FP64 mode)
+        sigma_sum<SI>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf);
         if ((N & me) && acc) {
-            const uint64_t old =
-                atomicAdd((unsigned long long *)&b.sig[(int64_t)v * 64 + lane], (unsigned long long)acc);
-            const uint64_t nw = old + acc;
-            if (ovf || nw < old || (b.limit && nw >= b.limit)) b.ctr[1] = 1;
+            const int64_t k = (int64_t)v * 64 + lane;
+            uint64_t old, nw;
+            if (SI == SIG_U32) {
+                old = atomicAdd((unsigned int *)b.sig + k, (unsigned int)acc);
+                nw = old + acc;
+            } else {
+                old = atomicAdd((unsigned long long *)b.sig + k, (unsigned long long)acc);
+                nw = old + acc;
+                ovf |= nw < old;
+            }
+            if (ovf || acc > sg_max<SI>() || nw > sg_max<SI>() || (b.limit && nw >= b.limit)) b.ctr[1] = 1;
         }
     }
 }
@@ -354,7 +387,7 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
                     if (xx & me) {
                         use[u] = true;
                         t[u] = tt[(int64_t)ww * 64 + lane];
-                        if (DF) sw[u] = b.sig[(int64_t)ww * 64 + lane];
+                        if (DF) sw[u] = sg<SIG_F64>(b, (int64_t)ww * 64 + lane);
                     }
                 }
             }
@@ -371,14 +404,13 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
 
 // rho complete for (v, lanes M): delta, BC contribution and the term t_v.
 // `acc` is rho_v (rho form) or delta_v (successor form, see rho_sum).
-template <bool F64, typename TT>
+template <int SM, typename TT>
 __device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t M, double acc) {
-    constexpr bool DF = F64 && sizeof(TT) == 4;
+    constexpr bool DF = SM == SIG_F64 && sizeof(TT) == 4;
     const int lane = threadIdx.x & 63;
     double contrib = 0.0;
     if (M & (1ull << lane)) {
-        const uint64_t sx = b.sig[(int64_t)v * 64 + lane];
-        const double sv = F64 ? u2d(sx) : (double)sx;
+        const double sv = sg_val<SM>(sg<SM>(b, (int64_t)v * 64 + lane));
         const double delta = DF ? acc : sv * acc;
         contrib = b.ws[lane] * delta;
         ((TT *)b.tt)[(int64_t)v * 64 + lane] =
@@ -393,8 +425,9 @@ __device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t
 
 // Level l <= depth: entries Q_l; Q_{l+2} is cleared from mb[(l+2)%3] and Q_l
 // scattered into mb[l%3] for level l-1.
-template <bool F64, typename TT>
+template <int SM, typename TT>
 __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
+    constexpr bool F64 = SM == SIG_F64;
     __shared__ int t_w[NW][WIN];
     __shared__ uint64_t t_x[NW][WIN];
     __shared__ int t_k[NW][WIN];
@@ -503,7 +536,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                         if (t_x[wv][rr] & me) {
                             use[u] = true;
                             t[u] = tt[(int64_t)t_w[wv][rr] * 64 + lane];
-                            if (DF) sw[u] = b.sig[(int64_t)t_w[wv][rr] * 64 + lane];
+                            if (DF) sw[u] = sg<SIG_F64>(b, (int64_t)t_w[wv][rr] * 64 + lane);
                         }
                     }
                 }
@@ -519,7 +552,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
 #pragma unroll
                             for (int j = 1; j < FEB; ++j)
                                 if (kcur == j) vk = V[j];
-                            svcur = u2d(b.sig[(int64_t)vk * 64 + lane]);  // used only where lane is in the entry's mask
+                            svcur = u2d(sg<SIG_F64>(b, (int64_t)vk * 64 + lane));  // used only where lane is in the entry's mask
                         }
                     }
                     if (use[u]) {
@@ -534,14 +567,14 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         // finish every entry of the block (hubs excepted): sigma rows first
         uint64_t SX[FEB];
 #pragma unroll
-        for (int j = 0; j < FEB; ++j) SX[j] = (OWN[j] && (MM[j] & me)) ? b.sig[(int64_t)V[j] * 64 + lane] : 0;
+        for (int j = 0; j < FEB; ++j) SX[j] = (OWN[j] && (MM[j] & me)) ? sg<SM>(b, (int64_t)V[j] * 64 + lane) : 0;
 #pragma unroll
         for (int j = 0; j < FEB; ++j) {
             if (!OWN[j]) continue;
             const int v = V[j];
             double contrib = 0.0;
             if (MM[j] & me) {
-                const double sv = F64 ? u2d(SX[j]) : (double)SX[j];
+                const double sv = sg_val<SM>(SX[j]);
                 const double a = r_rho[wv][j][lane];  // rho_v, or delta_v in the successor form
                 const double delta = DF ? a : sv * a;
                 contrib = b.ws[lane] * delta;
@@ -561,7 +594,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         const uint64_t M = b.hub_mask[(int64_t)h * b.hub_stride + l];
         if (!M) continue;
         const int v = b.hub_v[h];
-        const double svh = DF && (M & (1ull << lane)) ? u2d(b.sig[(int64_t)v * 64 + lane]) : 0.0;
+        const double svh = DF && (M & (1ull << lane)) ? u2d(sg<SIG_F64>(b, (int64_t)v * 64 + lane)) : 0.0;
         const double part = rho_sum<TT, DF>(b, succ, M, b.sl_j0[s], b.sl_j1[s], svh);
         if ((M & (1ull << lane)) && part != 0.0) unsafeAtomicAdd(&b.hub_acc[h * 64 + lane], part);
         __threadfence();
@@ -571,7 +604,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         if (done != b.hub_nsl[h] - 1) continue;
         __threadfence();
         const double rho = __hip_atomic_load(&b.hub_acc[h * 64 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        finish_vertex<F64, TT>(b, v, M, rho);
+        finish_vertex<SM, TT>(b, v, M, rho);
         if (lane == 0) b.hub_cnt[h] = 0;
     }
 }
@@ -607,30 +640,33 @@ __global__ __launch_bounds__(BB) void k_b_fold(double *dst, double *src, int n) 
 }
 
 // ------------------------------------------------------------ launchers ---
-hipError_t launch_b_init(const BatchDev &b, int ns, bool f64, int cap, hipStream_t st) {
+hipError_t launch_b_init(const BatchDev &b, int ns, int sm, int cap, hipStream_t st) {
     int g = (int)std::min<int64_t>(((int64_t)b.n + BB - 1) / BB, cap);
-    hipLaunchKernelGGL(k_b_clear_vis, dim3(std::max(g, 1)), dim3(BB), 0, st, b);
-    if (f64) hipLaunchKernelGGL(k_b_seed<true>, dim3(1), dim3(64), 0, st, b, ns);
-    else hipLaunchKernelGGL(k_b_seed<false>, dim3(1), dim3(64), 0, st, b, ns);
+    hipLaunchKernelGGL(k_b_clear_vis, dim3(std::max(g, 1)), dim3(BB), 0, st, b, sm);
+    if (sm == SIG_U32) hipLaunchKernelGGL(k_b_seed<SIG_U32>, dim3(1), dim3(64), 0, st, b, ns);
+    else if (sm == SIG_U64) hipLaunchKernelGGL(k_b_seed<SIG_U64>, dim3(1), dim3(64), 0, st, b, ns);
+    else hipLaunchKernelGGL(k_b_seed<SIG_F64>, dim3(1), dim3(64), 0, st, b, ns);
     return hipGetLastError();
 }
 
-hipError_t launch_b_forward(const BatchDev &b, bool f64, int l, int grid, hipStream_t st) {
+hipError_t launch_b_forward(const BatchDev &b, int sm, int l, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_b_expand, dim3(grid), dim3(BB), 0, st, b, l);
-    if (f64) hipLaunchKernelGGL(k_b_pull<true>, dim3(grid), dim3(BB), 0, st, b, l);
-    else hipLaunchKernelGGL(k_b_pull<false>, dim3(grid), dim3(BB), 0, st, b, l);
+    if (sm == SIG_U32) hipLaunchKernelGGL(k_b_pull<SIG_U32>, dim3(grid), dim3(BB), 0, st, b, l);
+    else if (sm == SIG_U64) hipLaunchKernelGGL(k_b_pull<SIG_U64>, dim3(grid), dim3(BB), 0, st, b, l);
+    else hipLaunchKernelGGL(k_b_pull<SIG_F64>, dim3(grid), dim3(BB), 0, st, b, l);
     return hipGetLastError();
 }
 
-hipError_t launch_b_back(const BatchDev &b, bool f64, bool fp32, int l, int grid, hipStream_t st) {
-    const dim3 g(grid);
-    if (f64) {
-        if (fp32) hipLaunchKernelGGL((k_b_back<true, float>), g, dim3(BB), 0, st, b, l);
-        else hipLaunchKernelGGL((k_b_back<true, double>), g, dim3(BB), 0, st, b, l);
-    } else {
-        if (fp32) hipLaunchKernelGGL((k_b_back<false, float>), g, dim3(BB), 0, st, b, l);
-        else hipLaunchKernelGGL((k_b_back<false, double>), g, dim3(BB), 0, st, b, l);
-    }
+template <int SM>
+static void back_tt(const BatchDev &b, bool fp32, int l, int grid, hipStream_t st) {
+    if (fp32) hipLaunchKernelGGL((k_b_back<SM, float>), dim3(grid), dim3(BB), 0, st, b, l);
+    else hipLaunchKernelGGL((k_b_back<SM, double>), dim3(grid), dim3(BB), 0, st, b, l);
+}
+
+hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st) {
+    if (sm == SIG_U32) back_tt<SIG_U32>(b, fp32, l, grid, st);
+    else if (sm == SIG_U64) back_tt<SIG_U64>(b, fp32, l, grid, st);
+    else back_tt<SIG_F64>(b, fp32, l, grid, st);
     return hipGetLastError();
 }
 

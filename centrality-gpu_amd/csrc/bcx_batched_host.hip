@@ -1,7 +1,8 @@
 // Host driver of the batched multi-source path: tiles of up to 64 sources
 // of one component, forward level lists built on the device, FP32-compensated
-// (or FP64) rho-form backward, sigma overflow handled per tile by redoing the
-// tile with FP64 sigma.  Sources are ordered by a BFS from the component's
+// (or FP64) rho-form backward.  Path counts start in 32-bit words (half the
+// bytes of every sigma gather); a tile whose counts leave the word's range
+// is redone with 64-bit words, and past those with FP64.  Sources are ordered by a BFS from the component's
 // highest-degree vertex so the 64 sources of a tile are close together and
 // their frontiers overlap (fewer (vertex, level) list entries per source).
 //
@@ -33,7 +34,7 @@ struct Slot {
     int32_t *pinned = nullptr;
     // tile in flight
     int tile = -1, ns = 0, L = 0, C = 0, attempts = 0;
-    bool f64 = false;
+    int sm = SIG_U32;  // sigma mode of the tile in flight
 };
 
 struct BatchedState {
@@ -43,6 +44,7 @@ struct BatchedState {
     int32_t src_cap = 0, n_hub = 0, n_slice = 0, hub_stride = 0;
     bool probe = false;
     int last_depth = 8;
+    int last_sm = SIG_U32;  // sigma mode the last tile finished in: the next starts there
 };
 
 static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
@@ -112,7 +114,7 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uin
     b.mb0 = (uint64_t *)s.mb.p;
     b.mb1 = b.mb0 + s.n_cap;
     b.mb2 = b.mb1 + s.n_cap;
-    b.sig = (uint64_t *)s.sig.p;
+    b.sig = s.sig.p;
     b.tt = s.tt.p;
     b.lvl = bs.probe ? (uint16_t *)s.lvl.p : nullptr;
     b.dbg_delta = bs.probe ? (double *)s.dbg.p : nullptr;
@@ -153,13 +155,13 @@ struct TileJob {
 // the depth or of an unfinished forward return at once), the cleanup, the
 // counter read-back and the completion event.
 static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st) {
-    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, s.f64 ? 0 : j.limit);
+    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, s.sm == SIG_F64 ? 0 : j.limit);
     int rc;
     for (int l = s.L; l < s.L + s.C; ++l)
-        if ((rc = ctx_hip(ctx, launch_b_forward(b, s.f64, l, j.grid, s.stream), "forward"))) return rc;
+        if ((rc = ctx_hip(ctx, launch_b_forward(b, s.sm, l, j.grid, s.stream), "forward"))) return rc;
     s.L += s.C;
     for (int l = s.L - 1; l >= 1; --l)
-        if ((rc = ctx_hip(ctx, launch_b_back(b, s.f64, j.fp32, l, j.grid, s.stream), "back"))) return rc;
+        if ((rc = ctx_hip(ctx, launch_b_back(b, s.sm, j.fp32, l, j.grid, s.stream), "back"))) return rc;
     if ((rc = ctx_hip(ctx, launch_b_finish(b, j.grid, s.stream), "finish"))) return rc;
     if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1;
     if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.pinned, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream),
@@ -168,16 +170,19 @@ static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st)
     return ctx_hip(ctx, hipEventRecord(s.done, s.stream), "event");
 }
 
-static int start_tile(bcx_ctx *ctx, Slot &s, int tile, int ns, bool f64, const TileJob &j, bcx_stats *st) {
+static int start_tile(bcx_ctx *ctx, Slot &s, int tile, int ns, int sm, const TileJob &j, bcx_stats *st) {
     s.tile = tile;
     s.ns = ns;
-    s.f64 = f64;
+    s.sm = sm;
     s.L = 0;
     s.C = std::max(4, ctx->batched->last_depth + 3);
-    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, f64 ? 0 : j.limit);
-    int rc = ctx_hip(ctx, launch_b_init(b, ns, f64, j.grid, s.stream), "init");
+    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, sm == SIG_F64 ? 0 : j.limit);
+    int rc = ctx_hip(ctx, launch_b_init(b, ns, sm, j.grid, s.stream), "init");
     if (rc) return rc;
-    if (st) st->n_launches += 2;
+    if (st) {
+        st->n_launches += 2;
+        if (sm == SIG_F64) st->n_f64_sigma += ns;
+    }
     return enqueue_chunk(ctx, s, j, st);
 }
 
@@ -208,21 +213,24 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
             return 0;
         }
         s.qcap = (int32_t)q2;
-        *rc_out = start_tile(ctx, s, s.tile, s.ns, s.f64, j, st);
+        if (st && s.sm == SIG_F64) st->n_f64_sigma -= s.ns;  // counted again by the restart
+        *rc_out = start_tile(ctx, s, s.tile, s.ns, s.sm, j, st);
         return *rc_out ? 0 : 1;
     }
-    if (!s.f64 && h[1]) {
-        // uint64 sigma overflowed: the backward did not run; redo in FP64
+    if (s.sm != SIG_F64 && h[1]) {
+        // a count left the word's range: the backward did not run; redo the
+        // tile with the next wider sigma word
         BatchDev b = make_dev(ctx, s, *j.dc, j.bc, 0);
         if ((rc = ctx_hip(ctx, launch_b_clear_all(b, std::min(h[0], s.qcap), j.grid, s.stream), "clear"))) {
             *rc_out = rc;
             return 0;
         }
-        if (st) st->n_f64_sigma += s.ns;
-        *rc_out = start_tile(ctx, s, s.tile, s.ns, true, j, st);
+        s.attempts = 0;
+        *rc_out = start_tile(ctx, s, s.tile, s.ns, s.sm + 1, j, st);
         return *rc_out ? 0 : 1;
     }
     ctx->batched->last_depth = h[3];
+    ctx->batched->last_sm = s.sm;
     if (st && h[3] > st->max_level) st->max_level = h[3];
     return 0;
 }
@@ -230,7 +238,7 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
 // Runs tiles [0, ntiles) over the slots; kernel time is the span between
 // events on the context stream that every slot stream is ordered against.
 static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob &j, bcx_stats *st,
-                     bool *f64_out) {
+                     int *sm_out) {
     BatchedState &bs = *ctx->batched;
     const int64_t ntiles = (n_src + BATCH_W - 1) / BATCH_W;
     const int ns_slots = (int)std::min<int64_t>(bs.n_slots, ntiles);
@@ -242,7 +250,8 @@ static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob
     for (int i = 0; i < ns_slots; ++i) (void)hipStreamWaitEvent(bs.slots[i].stream, e0, 0);
     int64_t next = 0;
     int active = 0;
-    bool any_f64 = false;
+    int max_sm = SIG_U32;
+    bs.last_sm = SIG_U32;
     for (int i = 0; i < ns_slots; ++i) bs.slots[i].tile = -1;
     rc = BCX_OK;
     while ((next < ntiles || active > 0) && !rc) {
@@ -252,14 +261,17 @@ static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob
                 if (next >= ntiles) continue;
                 s.attempts = 0;
                 const int ns = (int)std::min<int64_t>(BATCH_W, n_src - next * BATCH_W);
-                rc = start_tile(ctx, s, (int)next++, ns, false, j, st);
+                // start in the mode the last tile needed (neighbouring tiles
+                // have similar counts); every 16th tile tries 32-bit words again
+                const int sm = (next % 16 == 0) ? SIG_U32 : bs.last_sm;
+                rc = start_tile(ctx, s, (int)next++, ns, sm, j, st);
                 active++;
                 continue;
             }
             int r2;
             if (poll_tile(ctx, s, j, st, &r2) == 0) {
                 rc = r2;
-                any_f64 |= s.f64;
+                max_sm = std::max(max_sm, s.sm);
                 s.tile = -1;
                 active--;
             }
@@ -275,7 +287,7 @@ static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob
     ctx->timer.end(ctx);
     for (int i = 0; i < ns_slots; ++i) (void)hipStreamSynchronize(bs.slots[i].stream);
     (void)hipEventDestroy(e0);
-    if (f64_out) *f64_out = any_f64;
+    if (sm_out) *sm_out = max_sm;
     return rc;
 }
 
@@ -559,7 +571,7 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
     const int32_t n = dc.n;
     rc = bs_ensure(ctx, n, BATCH_W, 1, true);
     if (!rc) rc = ctx->bc_acc.ensure(ctx, sizeof(double) * n, "bc accumulator");
-    bool f64 = false;
+    int sm = SIG_U32;
     std::vector<uint16_t> lv((size_t)n * 64);
     std::vector<uint64_t> sg((size_t)n * 64);
     std::vector<double> dl((size_t)n * 64);
@@ -567,13 +579,14 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
         std::vector<int32_t> tile{src};
         rc = stage_sources(ctx, dc, tile);
         if (!rc) rc = stage_hubs(ctx, dc);
-        if (!rc) rc = run_tiles(ctx, dc, 1, make_job(ctx, dc, o, (double *)ctx->bc_acc.p), nullptr, &f64);
+        if (!rc) rc = run_tiles(ctx, dc, 1, make_job(ctx, dc, o, (double *)ctx->bc_acc.p), nullptr, &sm);
     }
     Slot &s = ctx->batched->slots[0];
     // copies are ordered on the slot stream the kernels ran on
     if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), s.lvl.p, 2 * lv.size(), hipMemcpyDeviceToHost, s.stream),
                           "copy");
-    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), s.sig.p, 8 * sg.size(), hipMemcpyDeviceToHost, s.stream),
+    const size_t wsz = sm == SIG_U32 ? 4 : 8;
+    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), s.sig.p, wsz * sg.size(), hipMemcpyDeviceToHost, s.stream),
                           "copy");
     if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), s.dbg.p, 8 * dl.size(), hipMemcpyDeviceToHost, s.stream),
                           "copy");
@@ -582,6 +595,13 @@ int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, c
     ctx->batched->probe = false;
     bcx_dgraph_free(ctx, g);
     if (rc) return rc;
+    const bool f64 = sm == SIG_F64;
+    if (sm == SIG_U32) {  // 32-bit words: widen lane 0 of every vertex
+        std::vector<uint64_t> w((size_t)n * 64, 0);
+        const uint32_t *sg32 = (const uint32_t *)sg.data();
+        for (size_t i = 0; i < w.size(); i += 64) w[i] = sg32[i];
+        sg.swap(w);
+    }
     for (int32_t v = 0; v < n; ++v) {
         const size_t i = (size_t)v * 64;
         if (level) level[v] = lv[i];
