@@ -128,7 +128,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--slices", type=int, default=64)
+    ap.add_argument("--slices", type=int, default=0,
+                    help="0 = per workload: 64 for the wide graphs, 1 for the deep grid (4)")
     ap.add_argument("--plain-slices", type=int, default=1024)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-sources", type=int, default=0, help="0 = 4 per host thread")
@@ -144,6 +145,10 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if not args.slices:
+        # a step must hold enough tiles to fill the GPU: the deep grid's
+        # tiles run one workgroup each, so its step is the whole job
+        args.slices = {1: 1, 4: 1}.get(args.config, 64)
     t0 = time.perf_counter()
     n, uv = bcx.generate(args.config, 0, args.scale)
     host = bcx.Host.from_edges(n, uv).decompose()
@@ -164,7 +169,7 @@ def main():
 
     # batched FP32-compensated path (the headline)
     K = args.steps
-    idx = list(range(K))
+    idx = [i % args.slices for i in range(K)]
     t_b, tk_b, st_b = timed_steps(g, slices, idx, args.warmup, path=bcx.PATH_BATCHED,
                                   precision=bcx.PREC_FP32C, use_chains=True, batch_width=args.slots)
     src_b = sum(len(slices[i][0]) for i in idx)
@@ -176,7 +181,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        pidx = list(range(args.plain_steps))
+        pidx = [i % args.plain_slices for i in range(args.plain_steps)]
         t_p, tk_p, st_p = timed_steps(g, pslices, pidx, 0, path=bcx.PATH_PLAIN,
                                       precision=bcx.PREC_FP64, use_chains=True)
         src_p = sum(len(pslices[i][0]) for i in pidx)

@@ -266,6 +266,8 @@ class Host:
 
 FLAG_NO_SMALL_KERNEL = 1
 FLAG_NO_GRAPH = 2
+FLAG_NO_WG_KERNEL = 4
+FLAG_WG_KERNEL = 8
 
 
 def make_opts(path=PATH_PLAIN, precision=PREC_FP64, use_chains=True, subset=None,

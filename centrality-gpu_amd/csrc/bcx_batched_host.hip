@@ -370,6 +370,36 @@ static std::vector<int32_t> bfs_rank(const HostComp &hc) {
     return rank;
 }
 
+// Narrow levels: the average BFS level width from the highest-degree vertex
+// is at most WG_WIDTH vertices (grids, meshes, long cycles), so the
+// multi-launch tiles would be dominated by per-level launch latency.
+constexpr int WG_WIDTH = 4096;
+
+bool batched_narrow(DComp &dc) {
+    if (dc.depth < 0) {
+        const HostComp &hc = dc.host;
+        int32_t root = 0;
+        for (int32_t v = 1; v < hc.n; ++v)
+            if (hc.rp[v + 1] - hc.rp[v] > hc.rp[root + 1] - hc.rp[root]) root = v;
+        std::vector<int32_t> lv((size_t)hc.n, -1), q;
+        q.reserve(hc.n);
+        q.push_back(root);
+        lv[root] = 0;
+        int32_t depth = 0;
+        for (size_t i = 0; i < q.size(); ++i) {
+            const int32_t v = q[i];
+            depth = std::max(depth, lv[v]);
+            for (int32_t j = hc.rp[v]; j < hc.rp[v + 1]; ++j)
+                if (lv[hc.col[j]] < 0) {
+                    lv[hc.col[j]] = lv[v] + 1;
+                    q.push_back(hc.col[j]);
+                }
+        }
+        dc.depth = depth;
+    }
+    return (int64_t)dc.n <= (int64_t)WG_WIDTH * (dc.depth + 1);
+}
+
 static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double *bc) {
     TileJob j;
     j.dc = &dc;
@@ -397,6 +427,43 @@ std::vector<int32_t> batched_sources(const DComp &dc, const Plan &plan) {
     return srcs;
 }
 
+// Tiles as graph balls: each tile is the 64 requested sources nearest (BFS
+// over the whole component) to the first unassigned source in `srcs`
+// order.  On deep, narrow graphs (grids) the sources of a ball are within a
+// few hops of each other, so a vertex is reached at few distinct levels per
+// tile; consecutive BFS ranks instead lie along one long BFS layer.
+std::vector<int32_t> ball_sources(const DComp &dc, const std::vector<int32_t> &srcs) {
+    const HostComp &hc = dc.host;
+    std::vector<int8_t> want((size_t)hc.n, 0);
+    for (int32_t s : srcs) want[s] = 1;
+    std::vector<int32_t> stamp((size_t)hc.n, -1), q, out;
+    q.reserve(hc.n);
+    out.reserve(srcs.size());
+    int32_t tile = 0;
+    for (int32_t u : srcs) {
+        if (want[u] != 1) continue;
+        size_t got = 0;
+        q.clear();
+        q.push_back(u);
+        stamp[u] = tile;
+        for (size_t i = 0; i < q.size() && got < (size_t)BATCH_W; ++i) {
+            const int32_t v = q[i];
+            if (want[v] == 1) {
+                want[v] = 2;
+                out.push_back(v);
+                ++got;
+            }
+            for (int32_t j = hc.rp[v]; j < hc.rp[v + 1]; ++j)
+                if (stamp[hc.col[j]] != tile) {
+                    stamp[hc.col[j]] = tile;
+                    q.push_back(hc.col[j]);
+                }
+        }
+        ++tile;
+    }
+    return out;
+}
+
 int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, const bcx_opts *o, double *bc,
                       bcx_stats *st) {
     const std::vector<int32_t> srcs = batched_sources(dc, plan);
@@ -412,7 +479,7 @@ int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, c
 // small components: one persistent launch over (component, 64-source) tiles
 // ------------------------------------------------------------------------
 struct SmallState {
-    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm;
+    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs;
     int grid = 0;
     // captured launch (counter reset kernel + persistent kernel), replayed
     // while its arguments stay the same
@@ -455,12 +522,19 @@ static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool f
     return ctx_hip(ctx, hipGraphLaunch(ss.exec, ctx->stream), "graph launch");
 }
 
-int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
-                  bcx_stats *st) {
+// One persistent launch over the tiles of `jobs`: the small-component kernel
+// (LDS state, n <= SMALL_N) or, with `wg`, the workgroup-per-tile kernel
+// (HBM slabs, n <= WG_N).  Tiles whose uint64 sigma overflows are rerun in
+// FP64 by a second launch.
+static int persistent_tiles(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+                            bcx_stats *st, bool wg) {
     if (jobs.empty()) return BCX_OK;
     if (!ctx->batched) ctx->batched = new BatchedState();
-    if (!ctx->small) ctx->small = new SmallState();
-    SmallState &ss = *ctx->small;
+    SmallState *&state = wg ? ctx->wgk : ctx->small;
+    if (!state) state = new SmallState();
+    SmallState &ss = *state;
+    int64_t nmax = 1;
+    for (const SmallJob &jb : jobs) nmax = std::max<int64_t>(nmax, jb.dc->n);
     std::vector<SmallComp> comps;
     std::vector<SmallTile> tiles;
     std::vector<int32_t> srcs;
@@ -495,13 +569,25 @@ int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opt
         (rc = ss.counter.ensure(ctx, 64, "small counters")) ||
         (rc = ss.redo.ensure(ctx, 4 * tiles.size(), "small redo list")))
         return rc;
-    // two workgroups per CU (56 KB of LDS each), never more than the tiles
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), 2 * (int64_t)ctx->n_cu));
-    const size_t slab = (size_t)SMALL_N * 64;
-    if ((rc = ss.sig.ensure(ctx, 8 * slab * grid, "small sigma")) ||
-        (rc = ss.tt.ensure(ctx, 8 * slab * grid, "small dependency terms")) ||
-        (rc = ss.Q.ensure(ctx, 4 * slab * grid, "small lists")) ||
-        (rc = ss.Qm.ensure(ctx, 8 * slab * grid, "small lists")))
+    // two small-kernel workgroups or one workgroup-kernel workgroup per CU,
+    // never more than the tiles, nor more slabs than fit in half the free
+    // capacity
+    const size_t slab = (size_t)(wg ? nmax : SMALL_N) * 64;
+    // list entries: a vertex joins at most one level per lane, so 64 n bounds a tile
+    const size_t qcap = wg ? (size_t)nmax * 64 + 64 : slab;
+    const size_t per_wg = 16 * slab + 12 * qcap + (wg ? 24 * (size_t)nmax + 4 * ((size_t)nmax + 2) : 0);
+    int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), (wg ? 1 : 2) * (int64_t)ctx->n_cu));
+    if (wg) {
+        const size_t room = ctx->mem_cap > ctx->mem_used ? (ctx->mem_cap - ctx->mem_used) / 2 : 0;
+        grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)(room / per_wg)));
+    }
+    if ((rc = ss.sig.ensure(ctx, 8 * slab * grid, "persistent sigma")) ||
+        (rc = ss.tt.ensure(ctx, 8 * slab * grid, "persistent dependency terms")) ||
+        (rc = ss.Q.ensure(ctx, 4 * qcap * grid, "persistent lists")) ||
+        (rc = ss.Qm.ensure(ctx, 8 * qcap * grid, "persistent lists")))
+        return rc;
+    if (wg && ((rc = ss.masks.ensure(ctx, 24 * (size_t)nmax * grid, "workgroup masks")) ||
+               (rc = ss.qoffs.ensure(ctx, 4 * ((size_t)nmax + 2) * grid, "workgroup level offsets"))))
         return rc;
     SmallArgs a{};
     a.comps = (const SmallComp *)ss.comps.p;
@@ -518,10 +604,19 @@ int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opt
     a.slab = (int64_t)slab;
     a.limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
                   ? (1ull << o->sigma_f64_threshold_log2) : 0;
+    a.masks = wg ? (uint64_t *)ss.masks.p : nullptr;
+    a.qoffs = wg ? (int32_t *)ss.qoffs.p : nullptr;
+    a.nmax = nmax;
+    a.qcap = (int64_t)qcap;
     const bool fp32 = o->precision == BCX_PREC_FP32C;
-    const bool use_graph = !(o->flags & BCX_FLAG_NO_GRAPH);
+    const bool use_graph = !wg && !(o->flags & BCX_FLAG_NO_GRAPH);
+    auto launch = [&](bool f64, bool graph) -> int {
+        if (!wg) return small_launch(ctx, ss, a, f64, fp32, (int)grid, graph);
+        int r = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "workgroup reset");
+        return r ? r : ctx_hip(ctx, launch_wg(a, f64, fp32, (int)grid, ctx->stream), "workgroup kernel");
+    };
     ctx->timer.begin(ctx);
-    if ((rc = small_launch(ctx, ss, a, false, fp32, grid, use_graph))) return rc;
+    if ((rc = launch(false, use_graph))) return rc;
     ctx->timer.end(ctx);
     if (st) st->n_launches += 2;
     int32_t h[2];
@@ -544,7 +639,7 @@ int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opt
         a.limit = 0;
         if (st) for (auto &t : again) st->n_f64_sigma += t.ns;
         ctx->timer.begin(ctx);
-        if ((rc = small_launch(ctx, ss, a, true, fp32, grid, false))) return rc;
+        if ((rc = launch(true, false))) return rc;
         ctx->timer.end(ctx);
         if (st) st->n_launches += 2;
         if ((rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel"))) return rc;
@@ -552,14 +647,30 @@ int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opt
     return BCX_OK;
 }
 
-void small_release(bcx_ctx *ctx) {
-    if (!ctx->small) return;
-    SmallState &ss = *ctx->small;
+int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+                  bcx_stats *st) {
+    return persistent_tiles(ctx, jobs, o, bc_base, st, false);
+}
+
+int batched_wg(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+               bcx_stats *st) {
+    return persistent_tiles(ctx, jobs, o, bc_base, st, true);
+}
+
+static void release_state(bcx_ctx *ctx, SmallState *&state) {
+    if (!state) return;
+    SmallState &ss = *state;
     if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
-    for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm})
+    for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm,
+                    &ss.masks, &ss.qoffs})
         b->release(ctx);
-    delete ctx->small;
-    ctx->small = nullptr;
+    delete state;
+    state = nullptr;
+}
+
+void small_release(bcx_ctx *ctx) {
+    release_state(ctx, ctx->small);
+    release_state(ctx, ctx->wgk);
 }
 
 int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o, int32_t *level,

@@ -45,6 +45,7 @@ struct DComp {
     int32_t n = 0;
     int64_t arcs = 0;
     int64_t offset = 0;  // into the concatenated BC vector
+    int32_t depth = -1;  // BFS depth from the highest-degree vertex (-1: not measured)
     HostComp host;
     DBuf b_rp, b_col, b_reach, b_chp, b_chn, b_che;
     CompDev dev{};
@@ -89,6 +90,7 @@ struct bcx_ctx {
     bcx::DBuf w_src, w_aux, w_mode, w_slots;
     BatchedState *batched = nullptr;
     SmallState *small = nullptr;
+    SmallState *wgk = nullptr;  // workgroup-per-tile kernel buffers
     std::vector<bcx::DBuf *> all_bufs() {
         return {&counters, &bc_acc, &plain_scratch, &rows_buf, &w_src, &w_aux, &w_mode, &w_slots};
     }
@@ -116,8 +118,12 @@ struct SmallJob {
     std::vector<int32_t> srcs;
 };
 std::vector<int32_t> batched_sources(const bcx::DComp &dc, const bcx::Plan &plan);
+std::vector<int32_t> ball_sources(const bcx::DComp &dc, const std::vector<int32_t> &srcs);
 int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
                   bcx_stats *st);
+int batched_wg(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+               bcx_stats *st);
+bool batched_narrow(bcx::DComp &dc);
 void small_release(bcx_ctx *ctx);
 int batched_component(bcx_ctx *ctx, bcx_dgraph *g, bcx::DComp &dc, const bcx::Plan &plan,
                       const bcx_opts *o, double *bc, bcx_stats *st);

@@ -575,6 +575,7 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         }
     }
     std::vector<SmallJob> small;  // batched path: components for the one-launch small kernel
+    std::vector<SmallJob> wg;     // batched path: deep components for the workgroup-per-tile kernel
     for (size_t ci = 0; ci < g->comps.size(); ++ci) {
         DComp &dc = g->comps[ci];
         const int32_t *sp = o.src_subset ? sub[ci].data() : nullptr;
@@ -590,11 +591,16 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
                         (double)plan.n_chain * (8.0 * dc.arcs + 100.0 * dc.n);
         s->teps_work += (double)(plan.n_bfs + plan.n_chain) * (double)dc.arcs * 0.5;
         if (o.path == BCX_PATH_PLAIN) rc = plain_component(ctx, dc, plan, &o, bc + dc.offset, s);
-        else if (dc.n <= SMALL_N && !(o.flags & 1)) small.push_back({&dc, batched_sources(dc, plan)});
+        else if (dc.n <= SMALL_N && !(o.flags & BCX_FLAG_NO_SMALL_KERNEL))
+            small.push_back({&dc, batched_sources(dc, plan)});
+        else if (dc.n <= WG_N && !(o.flags & BCX_FLAG_NO_WG_KERNEL) &&
+                 ((o.flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
+            wg.push_back({&dc, ball_sources(dc, batched_sources(dc, plan))});
         else rc = batched_component(ctx, g, dc, plan, &o, bc + dc.offset, s);
         if (rc) return rc;
     }
     if (!small.empty() && (rc = batched_small(ctx, small, &o, bc, s))) return rc;
+    if (!wg.empty() && (rc = batched_wg(ctx, wg, &o, bc, s))) return rc;
     rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "run");
     if (rc) return rc;
     s->t_chain_s = ctx->timer_chain.collect();

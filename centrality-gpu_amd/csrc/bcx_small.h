@@ -35,11 +35,23 @@ struct SmallArgs {
     void *tt;
     int32_t *Q;
     uint64_t *Qm;
-    int64_t slab;       // SMALL_N * 64
+    int64_t slab;       // sigma / term words per workgroup (largest n * 64)
     uint64_t limit;     // u64 sigma threshold (0 = 2^64)
+    // workgroup-per-tile kernel for larger components: the visited and level
+    // masks and the level offsets live in per-workgroup HBM slabs
+    uint64_t *masks;    // [grid][3 * nmax]
+    int32_t *qoffs;     // [grid][nmax + 2]
+    int64_t nmax;       // largest component
+    int64_t qcap;       // list entries per workgroup (64 * nmax + 64)
 };
 
 hipError_t launch_small(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st);
 hipError_t launch_small_reset(int32_t *counter, hipStream_t st);
+
+// Components up to WG_N vertices whose levels are narrow (deep BFS) run one
+// workgroup per tile through every level, like the small kernel, with the
+// per-vertex state in HBM.
+constexpr int WG_N = 1 << 17;
+hipError_t launch_wg(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st);
 
 }  // namespace bcx

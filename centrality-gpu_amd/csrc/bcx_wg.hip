@@ -1,0 +1,454 @@
+// Workgroup-per-tile kernel for components of up to WG_N vertices with
+// narrow levels (deep BFS: grids, meshes, long paths).  The multi-launch
+// tiles pay three launches per level, which a BFS hundreds of levels deep
+// turns into milliseconds of launch latency per tile; here one workgroup
+// takes a tile = (component, up to 64 sources) from a work counter and runs
+// every level of it with workgroup barriers, while hundreds of workgroups run
+// other tiles.
+//
+// Same method as the batched kernels (lane k of a wave64 owns source k): the
+// visited set and two level masks (u64 per vertex), sigma[v][64], the terms
+// t[v][64], the level lists and their offsets live in a per-workgroup HBM
+// slab.  Level l+1's list is appended by the first lane mask that reaches a
+// vertex (the atomic OR that finds zero), so no pass over all n vertices is
+// made per level.  Pull and backward walk the arcs of 8 list entries per wave
+// as one flat range (lane per arc), compact the arcs that matter in LDS and
+// then gather one sigma / term row per such arc, summed per entry in CSR
+// order (exact u64, FP64 in the oracle's order).  The backward is the rho
+// form in FP32 with Kahan compensation or FP64 (the successor form for FP64
+// sigma with FP32 terms), BC(v) += sum_k (r(s_k)+1) delta_k(v)
+// (Codes/kernels_free.cu:154-473 computes the same, one source per block).
+// A tile whose uint64 sigma overflows stops, adds nothing and is listed for
+// an FP64 rerun by the host.
+//
+// Mask words are OR-ed with L2 atomics while other waves of the workgroup
+// read them, so every read of a mask array that atomics write goes through
+// agent-scope loads (no stale vector-L1 line).
+#include "bcx_device.h"
+#include "bcx_small.h"
+
+namespace bcx {
+
+constexpr int GB = 1024;     // threads per workgroup (one workgroup per CU)
+constexpr int GW = GB / 64;  // waves per workgroup
+constexpr int GEB = 16;      // list entries per wave block
+constexpr int GWIN = 64;     // arcs per compaction window
+constexpr int GUNR = 8;      // row gathers in flight per lane
+
+struct KahanW {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x) {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+struct SumW {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
+// LDS of one wave: the block's entries (first flat arc index P, first CSR
+// arc A, vertex V, lane mask M), the compacted (endpoint, lanes, entry)
+// triples of a window and the per-entry sums (as 64-bit words).
+struct WaveLds {
+    int P[GEB], A[GEB], V[GEB];
+    uint64_t M[GEB];
+    int w[GWIN];
+    uint64_t x[GWIN];
+    int k[GWIN];
+    uint64_t r[GEB][64];
+};
+
+__device__ __forceinline__ uint64_t wsh64(uint64_t x, int src) { return (uint64_t)__shfl((long long)x, src, 64); }
+
+// Workgroup barrier after every outstanding memory access of the wave has
+// completed, so plain stores have reached L2 before other waves read the
+// words through L2 (agent-scope loads, atomics).
+__device__ __forceinline__ void wg_barrier() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+// Block layout in L: lane j < GEB describes entry j (vertex v, mask m, CSR
+// arcs from r0, d of them walked); P[j] = first flat arc index (INT32_MAX
+// past cnt).  Returns the block's flat arc count.
+__device__ __forceinline__ int block_layout(int d, int cnt, int r0, int v, uint64_t m, WaveLds &L) {
+    const int lane = threadIdx.x & 63;
+    int inc = d;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane < GEB) {
+        L.P[lane] = lane < cnt ? inc - d : INT32_MAX;
+        L.A[lane] = r0;
+        L.V[lane] = v;
+        L.M[lane] = m;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return __shfl(inc, 63, 64);
+}
+
+// Compacts the arcs [win, wend) of a block whose far-end mask & entry mask is
+// non-zero into L; returns the number of triples.
+__device__ __forceinline__ int compact(const int32_t *col, const uint64_t *far, int win, int wend, WaveLds &L) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t me = 1ull << lane;
+    int nrel = 0;
+    for (int c = win; c < wend; c += 64) {
+        const int g = c + lane;
+        int k = 0;
+#pragma unroll
+        for (int j = 1; j < GEB; ++j) k += (L.P[j] <= g);
+        int w = 0;
+        uint64_t x = 0;
+        if (g < wend) {
+            w = col[L.A[k] + g - L.P[k]];
+            x = ald(&far[w]) & L.M[k];
+        }
+        const uint64_t bal = __ballot(x != 0);
+        if (x) {
+            const int pos = nrel + __popcll(bal & (me - 1));
+            L.w[pos] = w;
+            L.x[pos] = x;
+            L.k[pos] = k;
+        }
+        nrel += __popcll(bal);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return nrel;
+}
+
+// Pull of list entries [e0, e0+cnt) of level l+1: lane masks (final after the
+// expand), sigma over predecessors in CSR order, visited update.
+template <bool F64>
+__device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *rp, const int32_t *col,
+                                           const uint64_t *cur, const uint64_t *nxt, const int32_t *Q, uint64_t *Qm,
+                                           uint64_t *sig, uint64_t *vis, int e0, int cnt, WaveLds &L, int *s_ovf) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t me = 1ull << lane;
+    int w0 = 0, r0 = 0, r1 = 0;
+    uint64_t N0 = 0;
+    if (lane < cnt) {
+        w0 = Q[e0 + lane];
+        N0 = ald(&nxt[w0]);
+        Qm[e0 + lane] = N0;
+        r0 = rp[w0];
+        r1 = rp[w0 + 1];
+    }
+    const int T = block_layout(lane < cnt ? r1 - r0 : 0, cnt, r0, w0, N0, L);
+#pragma unroll
+    for (int j = 0; j < GEB; ++j) L.r[j][lane] = 0;
+    uint64_t acc = 0;
+    double facc = 0.0;
+    bool ovf = false;
+    int kcur = -1;
+    for (int win = 0; win < T; win += GWIN) {
+        const int nrel = compact(col, cur, win, min(T, win + GWIN), L);
+        for (int r = 0; r < nrel; r += GUNR) {
+            uint64_t xv[GUNR];
+            int kk[GUNR];
+            bool use[GUNR];
+#pragma unroll
+            for (int u = 0; u < GUNR; ++u) {
+                const int rr = r + u;
+                use[u] = false;
+                kk[u] = -1;
+                xv[u] = 0;
+                if (rr < nrel) {
+                    kk[u] = L.k[rr];
+                    if (L.x[rr] & me) {
+                        use[u] = true;
+                        xv[u] = sig[(int64_t)L.w[rr] * 64 + lane];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < GUNR; ++u) {
+                if (kk[u] < 0) continue;
+                if (kk[u] != kcur) {
+                    if (kcur >= 0) L.r[kcur][lane] = F64 ? d2u(facc) : acc;
+                    acc = 0;
+                    facc = 0.0;
+                    kcur = kk[u];
+                }
+                if (!use[u]) continue;
+                if (F64) {
+                    facc += u2d(xv[u]);
+                } else {
+                    const uint64_t s = acc + xv[u];
+                    ovf |= (s < acc);
+                    acc = s;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (kcur >= 0) L.r[kcur][lane] = F64 ? d2u(facc) : acc;
+    __builtin_amdgcn_wave_barrier();
+    for (int j = 0; j < cnt; ++j) {
+        const int w = L.V[j];
+        const uint64_t N = L.M[j];
+        if (N & me) {
+            const uint64_t val = L.r[j][lane];
+            sig[(int64_t)w * 64 + lane] = val;
+            if (!F64 && limit && val >= limit) ovf = true;
+        }
+        if (lane == 0) vis[w] |= N;
+    }
+    if (!F64 && __ballot(ovf) && lane == 0) *s_ovf = 1;
+}
+
+// Backward of list entries [e0, e0+cnt) of level lb: scatter own masks, sum
+// successor terms (has_succ false at the deepest level), finish each entry.
+template <bool F64, typename TT>
+__device__ __forceinline__ void back_block(const int32_t *rp, const int32_t *col, const int32_t *reach, double *bc,
+                                           const double *ws, const uint64_t *succ, uint64_t *own, const int32_t *Q,
+                                           const uint64_t *Qm, const uint64_t *sig, TT *tt, int e0, int cnt,
+                                           bool has_succ, WaveLds &L) {
+    constexpr bool DF = F64 && sizeof(TT) == 4;  // successor form (see header)
+    using Acc = typename std::conditional<sizeof(TT) == 4, KahanW, SumW>::type;
+    const int lane = threadIdx.x & 63;
+    const uint64_t me = 1ull << lane;
+    int v0 = 0, r0 = 0, r1 = 0;
+    uint64_t M0 = 0;
+    if (lane < cnt) {
+        v0 = Q[e0 + lane];
+        M0 = Qm[e0 + lane];
+        own[v0] = M0;
+        r0 = rp[v0];
+        r1 = rp[v0 + 1];
+    }
+    const int T = block_layout(has_succ && lane < cnt ? r1 - r0 : 0, cnt, r0, v0, M0, L);
+#pragma unroll
+    for (int j = 0; j < GEB; ++j) L.r[j][lane] = d2u(0.0);
+    Acc acc;
+    int kcur = -1;
+    double svcur = 0.0;  // successor form: sigma of entry kcur on this lane
+    for (int win = 0; win < T; win += GWIN) {
+        const int nrel = compact(col, succ, win, min(T, win + GWIN), L);
+        for (int r = 0; r < nrel; r += GUNR) {
+            TT t[GUNR];
+            uint64_t sw[GUNR];
+            int kk[GUNR];
+            bool use[GUNR];
+#pragma unroll
+            for (int u = 0; u < GUNR; ++u) {
+                const int rr = r + u;
+                use[u] = false;
+                kk[u] = -1;
+                t[u] = 0;
+                sw[u] = 0;
+                if (rr < nrel) {
+                    kk[u] = L.k[rr];
+                    if (L.x[rr] & me) {
+                        use[u] = true;
+                        t[u] = tt[(int64_t)L.w[rr] * 64 + lane];
+                        if (DF) sw[u] = sig[(int64_t)L.w[rr] * 64 + lane];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < GUNR; ++u) {
+                if (kk[u] < 0) continue;
+                if (kk[u] != kcur) {
+                    if (kcur >= 0) L.r[kcur][lane] = d2u(acc.value());
+                    acc = Acc();
+                    kcur = kk[u];
+                    if (DF) svcur = u2d(sig[(int64_t)L.V[kcur] * 64 + lane]);  // read only where lane is in the mask
+                }
+                if (!use[u]) continue;
+                if (DF) acc.add((TT)(svcur / u2d(sw[u])) * t[u]);
+                else acc.add(t[u]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (kcur >= 0) L.r[kcur][lane] = d2u(acc.value());
+    __builtin_amdgcn_wave_barrier();
+    for (int j = 0; j < cnt; ++j) {
+        const int v = L.V[j];
+        double contrib = 0.0;
+        if (L.M[j] & me) {
+            const uint64_t sx = sig[(int64_t)v * 64 + lane];
+            const double sv = F64 ? u2d(sx) : (double)sx;
+            const double s = u2d(L.r[j][lane]);  // rho_v, or delta_v in the successor form
+            const double delta = DF ? s : sv * s;
+            contrib = ws[lane] * delta;
+            tt[(int64_t)v * 64 + lane] =
+                DF ? (TT)((double)reach[v] + 1.0 + delta) : (TT)(((double)reach[v] + 1.0) / sv + s);
+        }
+        contrib = wave_sum(contrib);
+        if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&bc[v], contrib);
+    }
+}
+
+template <bool F64, typename TT>
+__global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
+    __shared__ WaveLds lds[GW];
+    __shared__ int s_item, s_cnt, s_ovf;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t nm = a.nmax;
+    uint64_t *sig = a.sig + (int64_t)blockIdx.x * a.slab;
+    TT *tt = (TT *)a.tt + (int64_t)blockIdx.x * a.slab;
+    int32_t *Q = a.Q + (int64_t)blockIdx.x * a.qcap;
+    uint64_t *Qm = a.Qm + (int64_t)blockIdx.x * a.qcap;
+    uint64_t *vis = a.masks + (int64_t)blockIdx.x * 3 * nm;
+    uint64_t *mA = vis + nm, *mB = mA + nm;
+    int32_t *qoff = a.qoffs + (int64_t)blockIdx.x * (nm + 2);
+    while (true) {
+        if (tid == 0) s_item = atomicAdd(&a.counter[0], 1);
+        wg_barrier();
+        const int item = s_item;
+        if (item >= a.n_tiles) break;
+        const SmallTile tl = a.tiles[item];
+        const SmallComp cp = a.comps[tl.comp];
+        const int n = cp.n;
+        const int32_t *rp = cp.rp, *col = cp.col;
+        for (int v = tid; v < n; v += GB) {
+            vis[v] = 0;
+            mA[v] = 0;
+            mB[v] = 0;
+        }
+        if (tid == 0) {
+            s_cnt = 0;
+            s_ovf = 0;
+            qoff[0] = 0;
+            qoff[1] = tl.ns;
+        }
+        wg_barrier();
+        if (tid < tl.ns) {
+            const int s = a.srcs[tl.src0 + tid];
+            vis[s] = 1ull << tid;
+            mA[s] = 1ull << tid;
+            sig[(int64_t)s * 64 + tid] = F64 ? d2u(1.0) : 1ull;
+            Q[tid] = s;
+            Qm[tid] = 1ull << tid;
+        }
+        wg_barrier();
+        uint64_t *cur = mA, *nxt = mB;
+        int l = 0, q0 = 0, q1 = tl.ns;
+        // ------------------------------------------------------- forward
+        while (true) {
+            // expand: lane per entry, a whole wave for an entry of degree > 16;
+            // the OR that finds a zero word appends the vertex to Q_{l+1}
+            for (int base = q0 + wv * 64; base < q1; base += GB) {
+                const int e = base + lane;
+                int j0 = 0, j1 = 0;
+                uint64_t F = 0;
+                if (e < q1) {
+                    const int v = Q[e];
+                    F = Qm[e];
+                    j0 = rp[v];
+                    j1 = rp[v + 1];
+                }
+                if (j1 - j0 <= 16) {
+                    // four arcs at a time: their loads and ORs are independent
+                    for (int j = j0; j < j1; j += 4) {
+                        int w[4];
+                        uint64_t m[4], old[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) w[u] = j + u < j1 ? col[j + u] : -1;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) m[u] = w[u] >= 0 ? F & ~vis[w[u]] : 0;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            old[u] = m[u] ? atomicOr((unsigned long long *)&nxt[w[u]], (unsigned long long)m[u]) : 1;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (old[u] == 0) {
+                                const int pos = q1 + atomicAdd(&s_cnt, 1);
+                                if (pos < a.qcap) Q[pos] = w[u];
+                            }
+                    }
+                }
+                uint64_t bal = __ballot(j1 - j0 > 16);
+                while (bal) {
+                    const int src = __ffsll((long long)bal) - 1;
+                    bal &= bal - 1;
+                    const uint64_t FF = wsh64(F, src);
+                    const int a0 = __shfl(j0, src, 64), a1 = __shfl(j1, src, 64);
+                    for (int j = a0 + lane; j < a1; j += 64) {
+                        const int w = col[j];
+                        const uint64_t m = FF & ~vis[w];
+                        if (m && atomicOr((unsigned long long *)&nxt[w], (unsigned long long)m) == 0) {
+                            const int pos = q1 + atomicAdd(&s_cnt, 1);
+                            if (pos < a.qcap) Q[pos] = w;
+                        }
+                    }
+                }
+            }
+            wg_barrier();
+            const int q2 = min((int64_t)q1 + s_cnt, a.qcap);
+            wg_barrier();
+            if (tid == 0) {
+                qoff[l + 2] = q2;
+                s_cnt = 0;
+            }
+            if (q2 == q1) break;
+            // pull: sigma of Q_{l+1}, GEB entries per wave block
+            for (int e0 = q1 + wv * GEB; e0 < q2; e0 += GW * GEB)
+                pull_block<F64>(a.limit, rp, col, cur, nxt, Q, Qm, sig, vis, e0, min(GEB, q2 - e0), lds[wv], &s_ovf);
+            wg_barrier();
+            for (int e = q0 + tid; e < q1; e += GB) cur[Q[e]] = 0;
+            wg_barrier();
+            uint64_t *t = cur;
+            cur = nxt;
+            nxt = t;
+            l++;
+            q0 = q1;
+            q1 = q2;
+            if (s_ovf) break;  // uint64 overflow: the tile is rerun in FP64
+        }
+        const int D = l;  // deepest non-empty level
+        wg_barrier();
+        if (s_ovf) {
+            if (tid == 0) {
+                const int k = atomicAdd(&a.counter[1], 1);
+                a.redo[k] = item;
+            }
+            wg_barrier();
+            continue;
+        }
+        // ------------------------------------------------------ backward
+        // cur holds level D's masks (nxt is all zero): clear them so both
+        // arrays start at zero; each level scatters its own masks for the
+        // level above
+        for (int e = q0 + tid; e < q1; e += GB) cur[Q[e]] = 0;
+        wg_barrier();
+        uint64_t *succ = cur, *own = nxt;
+        for (int lb = D; lb >= 1; --lb) {
+            const int b0 = qoff[lb], b1 = qoff[lb + 1];
+            for (int e0 = b0 + wv * GEB; e0 < b1; e0 += GW * GEB)
+                back_block<F64, TT>(rp, col, cp.reach, cp.bc, a.ws + tl.src0, succ, own, Q, Qm, sig, tt, e0,
+                                    min(GEB, b1 - e0), lb < D, lds[wv]);
+            wg_barrier();
+            // level lb+1's masks are done with; level lb's become the successors
+            const int c0 = qoff[lb + 1], c1 = qoff[lb + 2];
+            for (int e = c0 + tid; e < c1; e += GB) succ[Q[e]] = 0;
+            wg_barrier();
+            uint64_t *t = succ;
+            succ = own;
+            own = t;
+        }
+        wg_barrier();
+    }
+}
+
+hipError_t launch_wg(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st) {
+    if (f64) {
+        if (fp32) hipLaunchKernelGGL((k_wg<true, float>), dim3(grid), dim3(GB), 0, st, a);
+        else hipLaunchKernelGGL((k_wg<true, double>), dim3(grid), dim3(GB), 0, st, a);
+    } else {
+        if (fp32) hipLaunchKernelGGL((k_wg<false, float>), dim3(grid), dim3(GB), 0, st, a);
+        else hipLaunchKernelGGL((k_wg<false, double>), dim3(grid), dim3(GB), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bcx

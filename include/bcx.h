@@ -149,6 +149,13 @@ typedef struct {
 /* Batched path: launch the small-component kernel eagerly instead of
  * replaying its captured hipGraph.                                         */
 #define BCX_FLAG_NO_GRAPH        2
+/* Batched path: components of 2049..131072 vertices whose BFS levels are
+ * narrow (average level width at most 4096 vertices from the highest-degree
+ * vertex) run one workgroup per tile through every level.  NO_WG_KERNEL
+ * sends them to the multi-launch tiles; WG_KERNEL sends every component of
+ * that size range to the workgroup kernel.                                 */
+#define BCX_FLAG_NO_WG_KERNEL    4
+#define BCX_FLAG_WG_KERNEL       8
 
 typedef struct {
     double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */

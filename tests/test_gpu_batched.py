@@ -180,3 +180,68 @@ def test_fp32c_sigma_beyond_f32_range(engine, flags):
     assert mx < 1e-6 and zeros
     for s in (0, k * L // 2):
         check_source(engine, c, s, bcx.PREC_FP32C)
+
+
+def grid_edges(w, h, base=0):
+    e = [(base + y * w + x, base + y * w + x + 1) for y in range(h) for x in range(w - 1)]
+    e += [(base + y * w + x, base + (y + 1) * w + x) for y in range(h - 1) for x in range(w)]
+    return e
+
+
+@pytest.mark.parametrize("prec,tol", [(bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)])
+def test_wg_kernel_deep_components(engine, prec, tol):
+    """Two grids (60x60 and 45x50, sigma past 2^64 in both) joined by a
+    bridge: both biconnected components run in one workgroup-per-tile launch
+    (uint64 pass, then the FP64 rerun of the overflowed tiles) and match the
+    oracle and the multi-launch tiles."""
+    e = grid_edges(60, 60) + grid_edges(45, 50, base=3600) + [(3599, 3600)]
+    n = 3600 + 45 * 50
+    h = bcx.Host.from_edges(n, e).decompose()
+    comps = h.components()
+    assert sorted(c.n for c in comps) == [2250, 3600]
+    rp, col = oracle.csr_from_edges(n, e)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    got, st = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_WG_KERNEL)
+    assert st["n_f64_sigma"] > 0
+    mx, mean, zeros = relerr(got, want)
+    print("wg prec", prec, "maxrel", mx, "launches", st["n_launches"], flush=True)
+    assert mx < tol and zeros
+    ref, _ = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_NO_WG_KERNEL)
+    mx2, _, _ = relerr(got, ref)
+    assert mx2 < 2 * tol
+
+
+@pytest.mark.parametrize("cfg,scale", [(2, 0.003), (4, 0.1), (5, 0.004)])
+def test_wg_kernel_generated(engine, cfg, scale):
+    """Forced onto the workgroup kernel, generated graphs (a power-law core
+    with hubs, the chained grid, R-MAT) match the oracle; the grid is also
+    picked by the narrow-level rule without the flag."""
+    n, uv = bcx.generate(cfg, 0, scale)
+    h = bcx.Host.from_edges(n, uv).decompose()
+    assert max(c.n for c in h.components()) > 2048
+    rp, col = oracle.csr_from_edges(n, uv)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    for prec, tol in ((bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)):
+        got, st = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_WG_KERNEL)
+        mx, mean, zeros = relerr(got, want)
+        print("cfg", cfg, "prec", prec, "maxrel", mx, "launches", st["n_launches"], flush=True)
+        assert mx < tol and zeros
+
+
+def test_wg_kernel_threshold_and_subset(engine):
+    """Forced 2^6 sigma threshold (FP64 rerun of every tile) and a source
+    subset on the workgroup kernel."""
+    c = comps_of(3600, grid_edges(60, 60))[0]
+    got, st = engine.bc([c], path=B, precision=bcx.PREC_FP32C, sigma_threshold_log2=6,
+                        flags=bcx.FLAG_WG_KERNEL)
+    assert st["n_f64_sigma"] == c.n
+    mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
+    assert mx < 1e-6 and zeros
+    g = engine.upload([c])
+    try:
+        sub = np.sort(np.random.default_rng(7).choice(c.n, size=500, replace=False))
+        got, st = g.run(path=B, precision=bcx.PREC_FP64, subset=sub, flags=bcx.FLAG_WG_KERNEL)
+        mx, _, zeros = relerr(got, oracle.component_bc(c.row_ptr, c.col, c.reach, sources=sub))
+        assert mx < 1e-9 and zeros
+    finally:
+        g.free()
