@@ -349,13 +349,9 @@ struct SumD {
     __device__ __forceinline__ double value() const { return s; }
 };
 
-// Sum over successors w (lanes in succ[w] & M) of the dependency term.
-// rho form (DF false): t_w, the sum is rho_v.  Successor form (DF true, FP64
-// sigma with f32 terms, where 1/sigma underflows f32): u_w = r(w)+1+delta_w
-// scaled by the FP64 ratio sigma_v/sigma_w rounded to f32, the sum is delta_v.
-template <typename TT, bool DF = false>
-__device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *succ, uint64_t M, int a0, int a1,
-                                          double sv = 0.0) {
+// rho_v = sum over successors w (lanes in succ[w] & M) of t_w.
+template <typename TT>
+__device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *succ, uint64_t M, int a0, int a1) {
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
     const TT *tt = (const TT *)b.tt;
     const int lane = threadIdx.x & 63;
@@ -372,13 +368,11 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
         uint64_t bal = __ballot(x != 0);
         while (bal) {
             TT t[UNR];
-            uint64_t sw[UNR];
             bool use[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 use[u] = false;
                 t[u] = 0;
-                sw[u] = 0;
                 if (bal) {
                     const int src = __ffsll((long long)bal) - 1;
                     bal &= bal - 1;
@@ -387,34 +381,27 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
                     if (xx & me) {
                         use[u] = true;
                         t[u] = tt[(int64_t)ww * 64 + lane];
-                        if (DF) sw[u] = sg<SIG_F64>(b, (int64_t)ww * 64 + lane);
                     }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                if (!use[u]) continue;
-                if (DF) acc.add((TT)(sv / u2d(sw[u])) * t[u]);
-                else acc.add(t[u]);
-            }
+            for (int u = 0; u < UNR; ++u)
+                if (use[u]) acc.add(t[u]);
         }
     }
     return acc.value();
 }
 
 // rho complete for (v, lanes M): delta, BC contribution and the term t_v.
-// `acc` is rho_v (rho form) or delta_v (successor form, see rho_sum).
 template <int SM, typename TT>
 __device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t M, double acc) {
-    constexpr bool DF = SM == SIG_F64 && sizeof(TT) == 4;
     const int lane = threadIdx.x & 63;
     double contrib = 0.0;
     if (M & (1ull << lane)) {
         const double sv = sg_val<SM>(sg<SM>(b, (int64_t)v * 64 + lane));
-        const double delta = DF ? acc : sv * acc;
+        const double delta = sv * acc;
         contrib = b.ws[lane] * delta;
-        ((TT *)b.tt)[(int64_t)v * 64 + lane] =
-            DF ? (TT)((double)b.reach[v] + 1.0 + delta) : (TT)(((double)b.reach[v] + 1.0) / sv + acc);
+        ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + acc);
         if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
     }
     contrib = wave_sum(contrib);
@@ -446,7 +433,6 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t me = 1ull << lane;
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
-    constexpr bool DF = F64 && sizeof(TT) == 4;  // successor form (see rho_sum)
     const TT *tt = (const TT *)b.tt;
     // Blocks of FEB list entries per wave.  The arcs of the block are walked
     // as one flat range (lane per arc) in windows of WIN: successor masks
@@ -489,7 +475,6 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         }
         Acc acc;
         int kcur = -1;
-        double svcur = 0.0;  // successor form: sigma of entry kcur on this lane
         for (int win = 0; win < T; win += WIN) {
             const int wend = min(T, win + WIN);
             int nrel = 0;
@@ -521,7 +506,6 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
             __builtin_amdgcn_wave_barrier();
             for (int r = 0; r < nrel; r += UNR) {
                 TT t[UNR];
-                uint64_t sw[UNR];
                 int kk[UNR];
                 bool use[UNR];
 #pragma unroll
@@ -530,13 +514,11 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                     use[u] = false;
                     kk[u] = -1;
                     t[u] = 0;
-                    sw[u] = 0;
                     if (rr < nrel) {
                         kk[u] = t_k[wv][rr];
                         if (t_x[wv][rr] & me) {
                             use[u] = true;
                             t[u] = tt[(int64_t)t_w[wv][rr] * 64 + lane];
-                            if (DF) sw[u] = sg<SIG_F64>(b, (int64_t)t_w[wv][rr] * 64 + lane);
                         }
                     }
                 }
@@ -547,18 +529,8 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                         if (kcur >= 0) r_rho[wv][kcur][lane] = acc.value();
                         acc = Acc();
                         kcur = kk[u];
-                        if (DF) {
-                            int vk = V[0];
-#pragma unroll
-                            for (int j = 1; j < FEB; ++j)
-                                if (kcur == j) vk = V[j];
-                            svcur = u2d(sg<SIG_F64>(b, (int64_t)vk * 64 + lane));  // used only where lane is in the entry's mask
-                        }
                     }
-                    if (use[u]) {
-                        if (DF) acc.add((TT)(svcur / u2d(sw[u])) * t[u]);
-                        else acc.add(t[u]);
-                    }
+                    if (use[u]) acc.add(t[u]);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -575,11 +547,10 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
             double contrib = 0.0;
             if (MM[j] & me) {
                 const double sv = sg_val<SM>(SX[j]);
-                const double a = r_rho[wv][j][lane];  // rho_v, or delta_v in the successor form
-                const double delta = DF ? a : sv * a;
+                const double a = r_rho[wv][j][lane];  // rho_v
+                const double delta = sv * a;
                 contrib = b.ws[lane] * delta;
-                ((TT *)b.tt)[(int64_t)v * 64 + lane] =
-                    DF ? (TT)((double)b.reach[v] + 1.0 + delta) : (TT)(((double)b.reach[v] + 1.0) / sv + a);
+                ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
                 if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
             }
             contrib = wave_sum(contrib);
@@ -594,8 +565,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         const uint64_t M = b.hub_mask[(int64_t)h * b.hub_stride + l];
         if (!M) continue;
         const int v = b.hub_v[h];
-        const double svh = DF && (M & (1ull << lane)) ? u2d(sg<SIG_F64>(b, (int64_t)v * 64 + lane)) : 0.0;
-        const double part = rho_sum<TT, DF>(b, succ, M, b.sl_j0[s], b.sl_j1[s], svh);
+        const double part = rho_sum<TT>(b, succ, M, b.sl_j0[s], b.sl_j1[s]);
         if ((M & (1ull << lane)) && part != 0.0) unsafeAtomicAdd(&b.hub_acc[h * 64 + lane], part);
         __threadfence();
         int done = 0;
@@ -663,10 +633,12 @@ static void back_tt(const BatchDev &b, bool fp32, int l, int grid, hipStream_t s
     else hipLaunchKernelGGL((k_b_back<SM, double>), dim3(grid), dim3(BB), 0, st, b, l);
 }
 
+// FP64-sigma tiles keep FP64 terms in either precision: 1/sigma can leave
+// the f32 range.
 hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st) {
     if (sm == SIG_U32) back_tt<SIG_U32>(b, fp32, l, grid, st);
     else if (sm == SIG_U64) back_tt<SIG_U64>(b, fp32, l, grid, st);
-    else back_tt<SIG_F64>(b, fp32, l, grid, st);
+    else hipLaunchKernelGGL((k_b_back<SIG_F64, double>), dim3(grid), dim3(BB), 0, st, b, l);
     return hipGetLastError();
 }
 

@@ -49,10 +49,6 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
     __shared__ int qoff[SMALL_N + 2];
     __shared__ int s_item, s_cnt, s_ovf;
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanS, SumS>::type;
-    // FP64 sigma with f32 terms: 1/sigma can underflow f32, so the terms are
-    // u_w = r(w)+1+delta_w scaled by the FP64 ratio sigma_v/sigma_w (<= 1) and
-    // the sum is delta_v itself (successor form) instead of rho_v.
-    constexpr bool DF = F64 && sizeof(TT) == 4;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint64_t me = 1ull << lane;
     uint64_t *sig = a.sig + (int64_t)blockIdx.x * a.slab;
@@ -232,7 +228,6 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
                 if (lane == 0) own[v] = M;
                 const int a0 = rp[v], a1 = rp[v + 1];
                 Acc acc;
-                const double svd = DF && (M & me) ? u2d(sig[(int64_t)v * 64 + lane]) : 0.0;
                 if (lb < D) {
                     for (int c = a0; c < a1; c += 64) {
                         const int j = c + lane;
@@ -245,13 +240,11 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
                         uint64_t bal = __ballot(x != 0);
                         while (bal) {
                             TT t[SUNR];
-                            uint64_t sw[SUNR];
                             bool use[SUNR];
 #pragma unroll
                             for (int u = 0; u < SUNR; ++u) {
                                 use[u] = false;
                                 t[u] = 0;
-                                sw[u] = 0;
                                 if (bal) {
                                     const int src = __ffsll((long long)bal) - 1;
                                     bal &= bal - 1;
@@ -259,16 +252,12 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
                                     if (sh64(x, src) & me) {
                                         use[u] = true;
                                         t[u] = tt[(int64_t)ww * 64 + lane];
-                                        if (DF) sw[u] = sig[(int64_t)ww * 64 + lane];
                                     }
                                 }
                             }
 #pragma unroll
-                            for (int u = 0; u < SUNR; ++u) {
-                                if (!use[u]) continue;
-                                if (DF) acc.add((TT)(svd / u2d(sw[u])) * t[u]);
-                                else acc.add(t[u]);
-                            }
+                            for (int u = 0; u < SUNR; ++u)
+                                if (use[u]) acc.add(t[u]);
                         }
                     }
                 }
@@ -276,11 +265,10 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
                 if (M & me) {
                     const uint64_t sx = sig[(int64_t)v * 64 + lane];
                     const double sv = F64 ? u2d(sx) : (double)sx;
-                    const double s = acc.value();  // rho_v, or delta_v in the successor form
-                    const double delta = DF ? s : sv * s;
+                    const double s = acc.value();  // rho_v
+                    const double delta = sv * s;
                     contrib = a.ws[tl.src0 + lane] * delta;
-                    tt[(int64_t)v * 64 + lane] =
-                        DF ? (TT)((double)cp.reach[v] + 1.0 + delta) : (TT)(((double)cp.reach[v] + 1.0) / sv + s);
+                    tt[(int64_t)v * 64 + lane] = (TT)(((double)cp.reach[v] + 1.0) / sv + s);
                 }
                 contrib = wave_sum(contrib);
                 if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&cp.bc[v], contrib);
@@ -298,10 +286,11 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
     }
 }
 
+// FP64-sigma tiles keep FP64 terms in either precision: 1/sigma can leave
+// the f32 range.
 hipError_t launch_small(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st) {
     if (f64) {
-        if (fp32) hipLaunchKernelGGL((k_small<true, float>), dim3(grid), dim3(SB), 0, st, a);
-        else hipLaunchKernelGGL((k_small<true, double>), dim3(grid), dim3(SB), 0, st, a);
+        hipLaunchKernelGGL((k_small<true, double>), dim3(grid), dim3(SB), 0, st, a);
     } else {
         if (fp32) hipLaunchKernelGGL((k_small<false, float>), dim3(grid), dim3(SB), 0, st, a);
         else hipLaunchKernelGGL((k_small<false, double>), dim3(grid), dim3(SB), 0, st, a);

@@ -15,15 +15,15 @@
 // as one flat range (lane per arc), compact the arcs that matter in LDS and
 // then gather one sigma / term row per such arc, summed per entry in CSR
 // order (exact u64, FP64 in the oracle's order).  The backward is the rho
-// form in FP32 with Kahan compensation or FP64 (the successor form for FP64
-// sigma with FP32 terms), BC(v) += sum_k (r(s_k)+1) delta_k(v)
+// form in FP32 with Kahan compensation or FP64 (always FP64 with FP64
+// sigma), BC(v) += sum_k (r(s_k)+1) delta_k(v)
 // (Codes/kernels_free.cu:154-473 computes the same, one source per block).
 // A tile whose uint64 sigma overflows stops, adds nothing and is listed for
 // an FP64 rerun by the host.
 //
-// Mask words are OR-ed with L2 atomics while other waves of the workgroup
-// read them, so every read of a mask array that atomics write goes through
-// agent-scope loads (no stale vector-L1 line).
+// Mask words (level masks and the visited set) are OR-ed with L2 atomics
+// while other waves of the workgroup read them, so every read of a mask
+// array goes through agent-scope loads (no stale vector-L1 line).
 #include "bcx_device.h"
 #include "bcx_small.h"
 
@@ -31,9 +31,10 @@ namespace bcx {
 
 constexpr int GB = 1024;     // threads per workgroup (one workgroup per CU)
 constexpr int GW = GB / 64;  // waves per workgroup
-constexpr int GEB = 16;      // list entries per wave block
-constexpr int GWIN = 64;     // arcs per compaction window
-constexpr int GUNR = 8;      // row gathers in flight per lane
+constexpr int GEB = 16;      // list entries per wave block (pull)
+constexpr int BEB = 8;       // list entries per wave block (backward)
+constexpr int GWIN = 128;    // arcs per compaction window (two 64-arc chunks)
+constexpr int GUNR = 8;      // row gathers in flight per lane (64-bit rows)
 
 struct KahanW {
     float s = 0.f, c = 0.f;
@@ -53,14 +54,14 @@ struct SumW {
 
 // LDS of one wave: the block's entries (first flat arc index P, first CSR
 // arc A, vertex V, lane mask M), the compacted (endpoint, lanes, entry)
-// triples of a window and the per-entry sums (as 64-bit words).
+// triples of a window and the backward's per-entry sums.
 struct WaveLds {
     int P[GEB], A[GEB], V[GEB];
     uint64_t M[GEB];
     int w[GWIN];
     uint64_t x[GWIN];
     int k[GWIN];
-    uint64_t r[GEB][64];
+    double r[BEB][64];
 };
 
 __device__ __forceinline__ uint64_t wsh64(uint64_t x, int src) { return (uint64_t)__shfl((long long)x, src, 64); }
@@ -94,60 +95,82 @@ __device__ __forceinline__ int block_layout(int d, int cnt, int r0, int v, uint6
     return __shfl(inc, 63, 64);
 }
 
+// Entry of flat arc index g: the last j with P[j] <= g (binary search).
+__device__ __forceinline__ int entry_of(const WaveLds &L, int g) {
+    int k = 0;
+#pragma unroll
+    for (int h = GEB / 2; h > 0; h >>= 1)
+        if (L.P[k + h] <= g) k += h;
+    return k;
+}
+
 // Compacts the arcs [win, wend) of a block whose far-end mask & entry mask is
-// non-zero into L; returns the number of triples.
+// non-zero into L, in arc order; returns the number of triples.  Both 64-arc
+// chunks of the window load their arcs and masks before either is compacted.
 __device__ __forceinline__ int compact(const int32_t *col, const uint64_t *far, int win, int wend, WaveLds &L) {
     const int lane = threadIdx.x & 63;
     const uint64_t me = 1ull << lane;
-    int nrel = 0;
-    for (int c = win; c < wend; c += 64) {
-        const int g = c + lane;
-        int k = 0;
-#pragma unroll
-        for (int j = 1; j < GEB; ++j) k += (L.P[j] <= g);
-        int w = 0;
-        uint64_t x = 0;
-        if (g < wend) {
-            w = col[L.A[k] + g - L.P[k]];
-            x = ald(&far[w]) & L.M[k];
-        }
-        const uint64_t bal = __ballot(x != 0);
-        if (x) {
-            const int pos = nrel + __popcll(bal & (me - 1));
-            L.w[pos] = w;
-            L.x[pos] = x;
-            L.k[pos] = k;
-        }
-        nrel += __popcll(bal);
+    const int g0 = win + lane, g1 = win + 64 + lane;
+    const int k0 = entry_of(L, g0), k1 = entry_of(L, g1);
+    int w0 = 0, w1 = 0;
+    uint64_t x0 = 0, x1 = 0;
+    if (g0 < wend) w0 = col[L.A[k0] + g0 - L.P[k0]];
+    if (g1 < wend) w1 = col[L.A[k1] + g1 - L.P[k1]];
+    if (g0 < wend) x0 = ald(&far[w0]) & L.M[k0];
+    if (g1 < wend) x1 = ald(&far[w1]) & L.M[k1];
+    const uint64_t b0 = __ballot(x0 != 0), b1 = __ballot(x1 != 0);
+    if (x0) {
+        const int pos = __popcll(b0 & (me - 1));
+        L.w[pos] = w0;
+        L.x[pos] = x0;
+        L.k[pos] = k0;
+    }
+    if (x1) {
+        const int pos = __popcll(b0) + __popcll(b1 & (me - 1));
+        L.w[pos] = w1;
+        L.x[pos] = x1;
+        L.k[pos] = k1;
     }
     __builtin_amdgcn_wave_barrier();
-    return nrel;
+    return __popcll(b0) + __popcll(b1);
 }
 
-// Pull of list entries [e0, e0+cnt) of level l+1: lane masks (final after the
-// expand), sigma over predecessors in CSR order, visited update.
+// Metadata of a block's entry on lane j < cnt (loaded a block ahead).
+struct EntryMeta {
+    int v, r0, r1;
+    uint64_t m;
+};
+
+// Pull of list entries [e0, e0+cnt) of level l+1 (lane j: entry j's vertex,
+// final lane mask and CSR range): sigma over predecessors in CSR order,
+// visited update.  Each entry is finished as soon as the walk moves past it
+// (every entry has at least one predecessor arc for each of its lanes).
 template <bool F64>
-__device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *rp, const int32_t *col,
-                                           const uint64_t *cur, const uint64_t *nxt, const int32_t *Q, uint64_t *Qm,
-                                           uint64_t *sig, uint64_t *vis, int e0, int cnt, WaveLds &L, int *s_ovf) {
+__device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *col, const uint64_t *cur, uint64_t *Qm,
+                                           uint64_t *sig, uint64_t *vis, int e0, int cnt, EntryMeta em, WaveLds &L,
+                                           int *s_ovf) {
     const int lane = threadIdx.x & 63;
     const uint64_t me = 1ull << lane;
-    int w0 = 0, r0 = 0, r1 = 0;
-    uint64_t N0 = 0;
-    if (lane < cnt) {
-        w0 = Q[e0 + lane];
-        N0 = ald(&nxt[w0]);
-        Qm[e0 + lane] = N0;
-        r0 = rp[w0];
-        r1 = rp[w0 + 1];
-    }
+    const int w0 = em.v, r0 = em.r0, r1 = em.r1;
+    const uint64_t N0 = em.m;
+    if (lane < cnt) Qm[e0 + lane] = N0;
     const int T = block_layout(lane < cnt ? r1 - r0 : 0, cnt, r0, w0, N0, L);
-#pragma unroll
-    for (int j = 0; j < GEB; ++j) L.r[j][lane] = 0;
     uint64_t acc = 0;
     double facc = 0.0;
     bool ovf = false;
     int kcur = -1;
+    auto finish = [&](int j) {
+        const int w = L.V[j];
+        const uint64_t N = L.M[j];
+        const uint64_t val = F64 ? d2u(facc) : acc;
+        if (N & me) {
+            sig[(int64_t)w * 64 + lane] = val;
+            if (!F64 && limit && val >= limit) ovf = true;
+        }
+        if (lane == 0) atomicOr((unsigned long long *)&vis[w], (unsigned long long)N);  // result unused: no return
+        acc = 0;
+        facc = 0.0;
+    };
     for (int win = 0; win < T; win += GWIN) {
         const int nrel = compact(col, cur, win, min(T, win + GWIN), L);
         for (int r = 0; r < nrel; r += GUNR) {
@@ -171,11 +194,9 @@ __device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *rp, co
 #pragma unroll
             for (int u = 0; u < GUNR; ++u) {
                 if (kk[u] < 0) continue;
-                if (kk[u] != kcur) {
-                    if (kcur >= 0) L.r[kcur][lane] = F64 ? d2u(facc) : acc;
-                    acc = 0;
-                    facc = 0.0;
-                    kcur = kk[u];
+                if (kk[u] != kcur) {  // wave-uniform
+                    if (kcur >= 0) finish(kcur);
+                    while (++kcur < kk[u]) finish(kcur);  // (entries without arcs: none in a pull)
                 }
                 if (!use[u]) continue;
                 if (F64) {
@@ -189,99 +210,80 @@ __device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *rp, co
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (kcur >= 0) L.r[kcur][lane] = F64 ? d2u(facc) : acc;
-    __builtin_amdgcn_wave_barrier();
-    for (int j = 0; j < cnt; ++j) {
-        const int w = L.V[j];
-        const uint64_t N = L.M[j];
-        if (N & me) {
-            const uint64_t val = L.r[j][lane];
-            sig[(int64_t)w * 64 + lane] = val;
-            if (!F64 && limit && val >= limit) ovf = true;
-        }
-        if (lane == 0) vis[w] |= N;
-    }
+    if (kcur >= 0) finish(kcur);
+    while (++kcur < cnt) finish(kcur);
     if (!F64 && __ballot(ovf) && lane == 0) *s_ovf = 1;
 }
 
-// Backward of list entries [e0, e0+cnt) of level lb: scatter own masks, sum
-// successor terms (has_succ false at the deepest level), finish each entry.
+// Backward of list entries [e0, e0+cnt) (cnt <= BEB) of level lb (lane j:
+// entry j's vertex, lane mask and CSR range): scatter own masks, sum
+// successor terms per entry into LDS (has_succ false at the deepest level),
+// then load the block's sigma words together and finish.
 template <bool F64, typename TT>
-__device__ __forceinline__ void back_block(const int32_t *rp, const int32_t *col, const int32_t *reach, double *bc,
-                                           const double *ws, const uint64_t *succ, uint64_t *own, const int32_t *Q,
-                                           const uint64_t *Qm, const uint64_t *sig, TT *tt, int e0, int cnt,
-                                           bool has_succ, WaveLds &L) {
-    constexpr bool DF = F64 && sizeof(TT) == 4;  // successor form (see header)
+__device__ __forceinline__ void back_block(const int32_t *col, const int32_t *reach, double *bc, const double *ws,
+                                           const uint64_t *succ, uint64_t *own, const uint64_t *sig, TT *tt, int cnt,
+                                           bool has_succ, EntryMeta em, WaveLds &L) {
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanW, SumW>::type;
+    constexpr int BUNR = sizeof(TT) == 8 ? GUNR : 2 * GUNR;  // 32-bit rows: twice as many in flight
     const int lane = threadIdx.x & 63;
     const uint64_t me = 1ull << lane;
-    int v0 = 0, r0 = 0, r1 = 0;
-    uint64_t M0 = 0;
-    if (lane < cnt) {
-        v0 = Q[e0 + lane];
-        M0 = Qm[e0 + lane];
-        own[v0] = M0;
-        r0 = rp[v0];
-        r1 = rp[v0 + 1];
-    }
+    const int v0 = em.v, r0 = em.r0, r1 = em.r1;
+    const uint64_t M0 = em.m;
+    if (lane < cnt) own[v0] = M0;
     const int T = block_layout(has_succ && lane < cnt ? r1 - r0 : 0, cnt, r0, v0, M0, L);
 #pragma unroll
-    for (int j = 0; j < GEB; ++j) L.r[j][lane] = d2u(0.0);
+    for (int j = 0; j < BEB; ++j) L.r[j][lane] = 0.0;
     Acc acc;
     int kcur = -1;
-    double svcur = 0.0;  // successor form: sigma of entry kcur on this lane
     for (int win = 0; win < T; win += GWIN) {
         const int nrel = compact(col, succ, win, min(T, win + GWIN), L);
-        for (int r = 0; r < nrel; r += GUNR) {
-            TT t[GUNR];
-            uint64_t sw[GUNR];
-            int kk[GUNR];
-            bool use[GUNR];
+        for (int r = 0; r < nrel; r += BUNR) {
+            TT t[BUNR];
+            int kk[BUNR];
+            bool use[BUNR];
 #pragma unroll
-            for (int u = 0; u < GUNR; ++u) {
+            for (int u = 0; u < BUNR; ++u) {
                 const int rr = r + u;
                 use[u] = false;
                 kk[u] = -1;
                 t[u] = 0;
-                sw[u] = 0;
                 if (rr < nrel) {
                     kk[u] = L.k[rr];
                     if (L.x[rr] & me) {
                         use[u] = true;
                         t[u] = tt[(int64_t)L.w[rr] * 64 + lane];
-                        if (DF) sw[u] = sig[(int64_t)L.w[rr] * 64 + lane];
                     }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < GUNR; ++u) {
+            for (int u = 0; u < BUNR; ++u) {
                 if (kk[u] < 0) continue;
                 if (kk[u] != kcur) {
-                    if (kcur >= 0) L.r[kcur][lane] = d2u(acc.value());
+                    if (kcur >= 0) L.r[kcur][lane] = acc.value();
                     acc = Acc();
                     kcur = kk[u];
-                    if (DF) svcur = u2d(sig[(int64_t)L.V[kcur] * 64 + lane]);  // read only where lane is in the mask
                 }
-                if (!use[u]) continue;
-                if (DF) acc.add((TT)(svcur / u2d(sw[u])) * t[u]);
-                else acc.add(t[u]);
+                if (use[u]) acc.add(t[u]);
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (kcur >= 0) L.r[kcur][lane] = d2u(acc.value());
+    if (kcur >= 0) L.r[kcur][lane] = acc.value();
     __builtin_amdgcn_wave_barrier();
-    for (int j = 0; j < cnt; ++j) {
+    uint64_t sx[BEB];
+#pragma unroll
+    for (int j = 0; j < BEB; ++j) sx[j] = (j < cnt && (L.M[j] & me)) ? sig[(int64_t)L.V[j] * 64 + lane] : 0;
+#pragma unroll
+    for (int j = 0; j < BEB; ++j) {
+        if (j >= cnt) break;
         const int v = L.V[j];
         double contrib = 0.0;
         if (L.M[j] & me) {
-            const uint64_t sx = sig[(int64_t)v * 64 + lane];
-            const double sv = F64 ? u2d(sx) : (double)sx;
-            const double s = u2d(L.r[j][lane]);  // rho_v, or delta_v in the successor form
-            const double delta = DF ? s : sv * s;
+            const double sv = F64 ? u2d(sx[j]) : (double)sx[j];
+            const double rho = L.r[j][lane];
+            const double delta = sv * rho;
             contrib = ws[lane] * delta;
-            tt[(int64_t)v * 64 + lane] =
-                DF ? (TT)((double)reach[v] + 1.0 + delta) : (TT)(((double)reach[v] + 1.0) / sv + s);
+            tt[(int64_t)v * 64 + lane] = (TT)(((double)reach[v] + 1.0) / sv + rho);
         }
         contrib = wave_sum(contrib);
         if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&bc[v], contrib);
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
 #pragma unroll
                         for (int u = 0; u < 4; ++u) w[u] = j + u < j1 ? col[j + u] : -1;
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) m[u] = w[u] >= 0 ? F & ~vis[w[u]] : 0;
+                        for (int u = 0; u < 4; ++u) m[u] = w[u] >= 0 ? F & ~ald(&vis[w[u]]) : 0;
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
                             old[u] = m[u] ? atomicOr((unsigned long long *)&nxt[w[u]], (unsigned long long)m[u]) : 1;
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
                     const int a0 = __shfl(j0, src, 64), a1 = __shfl(j1, src, 64);
                     for (int j = a0 + lane; j < a1; j += 64) {
                         const int w = col[j];
-                        const uint64_t m = FF & ~vis[w];
+                        const uint64_t m = FF & ~ald(&vis[w]);
                         if (m && atomicOr((unsigned long long *)&nxt[w], (unsigned long long)m) == 0) {
                             const int pos = q1 + atomicAdd(&s_cnt, 1);
                             if (pos < a.qcap) Q[pos] = w;
@@ -391,9 +393,37 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
                 s_cnt = 0;
             }
             if (q2 == q1) break;
-            // pull: sigma of Q_{l+1}, GEB entries per wave block
-            for (int e0 = q1 + wv * GEB; e0 < q2; e0 += GW * GEB)
-                pull_block<F64>(a.limit, rp, col, cur, nxt, Q, Qm, sig, vis, e0, min(GEB, q2 - e0), lds[wv], &s_ovf);
+            // pull: sigma of Q_{l+1}, GEB entries per wave block; a block's
+            // list entries are loaded two blocks ahead, their masks and CSR
+            // ranges one block ahead
+            {
+                constexpr int S = GW * GEB;
+                int e0 = q1 + wv * GEB;
+                int qn = (e0 + lane < q2 && lane < GEB) ? Q[e0 + lane] : 0;  // next block's vertices
+                EntryMeta em{};
+                auto meta = [&](int e, int v) {
+                    EntryMeta m{};
+                    if (e + lane < q2 && lane < GEB) {
+                        m.v = v;
+                        m.m = ald(&nxt[v]);
+                        m.r0 = rp[v];
+                        m.r1 = rp[v + 1];
+                    }
+                    return m;
+                };
+                if (e0 < q2) {
+                    em = meta(e0, qn);
+                    qn = (e0 + S + lane < q2 && lane < GEB) ? Q[e0 + S + lane] : 0;
+                }
+                for (; e0 < q2; e0 += S) {
+                    const EntryMeta cur_em = em;
+                    if (e0 + S < q2) {
+                        em = meta(e0 + S, qn);
+                        qn = (e0 + 2 * S + lane < q2 && lane < GEB) ? Q[e0 + 2 * S + lane] : 0;
+                    }
+                    pull_block<F64>(a.limit, col, cur, Qm, sig, vis, e0, min(GEB, q2 - e0), cur_em, lds[wv], &s_ovf);
+                }
+            }
             wg_barrier();
             for (int e = q0 + tid; e < q1; e += GB) cur[Q[e]] = 0;
             wg_barrier();
@@ -424,9 +454,40 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
         uint64_t *succ = cur, *own = nxt;
         for (int lb = D; lb >= 1; --lb) {
             const int b0 = qoff[lb], b1 = qoff[lb + 1];
-            for (int e0 = b0 + wv * GEB; e0 < b1; e0 += GW * GEB)
-                back_block<F64, TT>(rp, col, cp.reach, cp.bc, a.ws + tl.src0, succ, own, Q, Qm, sig, tt, e0,
-                                    min(GEB, b1 - e0), lb < D, lds[wv]);
+            {
+                // list entries two blocks ahead, CSR ranges one block ahead
+                constexpr int S = GW * BEB;
+                int e0 = b0 + wv * BEB;
+                auto entry = [&](int e, EntryMeta &m) {
+                    m = EntryMeta{};
+                    if (e + lane < b1 && lane < BEB) {
+                        m.v = Q[e + lane];
+                        m.m = Qm[e + lane];
+                    }
+                };
+                auto ranges = [&](int e, EntryMeta &m) {
+                    if (e + lane < b1 && lane < BEB) {
+                        m.r0 = rp[m.v];
+                        m.r1 = rp[m.v + 1];
+                    }
+                };
+                EntryMeta em{}, en{};
+                if (e0 < b1) {
+                    entry(e0, em);
+                    ranges(e0, em);
+                    entry(e0 + S, en);
+                }
+                for (; e0 < b1; e0 += S) {
+                    const EntryMeta cur_em = em;
+                    if (e0 + S < b1) {
+                        em = en;
+                        ranges(e0 + S, em);
+                        entry(e0 + 2 * S, en);
+                    }
+                    back_block<F64, TT>(col, cp.reach, cp.bc, a.ws + tl.src0, succ, own, sig, tt, min(BEB, b1 - e0),
+                                        lb < D, cur_em, lds[wv]);
+                }
+            }
             wg_barrier();
             // level lb+1's masks are done with; level lb's become the successors
             const int c0 = qoff[lb + 1], c1 = qoff[lb + 2];
@@ -440,10 +501,11 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
     }
 }
 
+// FP64-sigma tiles keep FP64 terms in either precision: 1/sigma can leave
+// the f32 range.
 hipError_t launch_wg(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st) {
     if (f64) {
-        if (fp32) hipLaunchKernelGGL((k_wg<true, float>), dim3(grid), dim3(GB), 0, st, a);
-        else hipLaunchKernelGGL((k_wg<true, double>), dim3(grid), dim3(GB), 0, st, a);
+        hipLaunchKernelGGL((k_wg<true, double>), dim3(grid), dim3(GB), 0, st, a);
     } else {
         if (fp32) hipLaunchKernelGGL((k_wg<false, float>), dim3(grid), dim3(GB), 0, st, a);
         else hipLaunchKernelGGL((k_wg<false, double>), dim3(grid), dim3(GB), 0, st, a);
