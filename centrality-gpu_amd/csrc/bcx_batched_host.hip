@@ -411,19 +411,29 @@ static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double
     return j;
 }
 
-// Every requested source of the plan runs as a BFS lane (its BFS sources and
-// its chain sources), ordered by BFS rank from the highest-degree vertex.
-std::vector<int32_t> batched_sources(const DComp &dc, const Plan &plan) {
-    std::vector<int32_t> srcs(plan.free_src);
-    for (auto &b : plan.batches) {
-        for (size_t i = 0; i < b.bsrc.size(); ++i)
-            if (b.bmode[i] & MODE_BC) srcs.push_back(b.bsrc[i]);
-        for (size_t i = 0; i < b.cci.size(); ++i)
-            srcs.push_back(dc.host.ch_nodes[dc.host.ch_ptr[b.cci[i]] + b.cpos[i]]);
+// Every requested source (the subset, or every vertex) runs as a BFS lane,
+// chain vertices included, ordered by BFS rank from the highest-degree
+// vertex.  *n_chain (when given) counts the chain vertices among them.
+std::vector<int32_t> batched_sources(DComp &dc, const int32_t *subset, int64_t n_sub, int64_t *n_chain) {
+    if (dc.rank.empty()) dc.rank = bfs_rank(dc.host);
+    std::vector<int32_t> srcs;
+    if (subset) {
+        srcs.assign(subset, subset + n_sub);
+        std::sort(srcs.begin(), srcs.end(), [&](int32_t a, int32_t b) { return dc.rank[a] < dc.rank[b]; });
+        srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
+    } else {
+        srcs.resize((size_t)dc.n);
+        for (int32_t v = 0; v < dc.n; ++v) srcs[(size_t)dc.rank[v]] = v;
     }
-    const std::vector<int32_t> rank = bfs_rank(dc.host);
-    std::sort(srcs.begin(), srcs.end(), [&](int32_t a, int32_t b) { return rank[a] < rank[b]; });
-    srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
+    if (n_chain) {
+        if (dc.on_chain.empty()) {
+            dc.on_chain.assign((size_t)dc.n, 0);
+            for (int32_t v : dc.host.ch_nodes) dc.on_chain[v] = 1;
+        }
+        int64_t c = 0;
+        for (int32_t v : srcs) c += dc.on_chain[v];
+        *n_chain = c;
+    }
     return srcs;
 }
 
@@ -464,9 +474,8 @@ std::vector<int32_t> ball_sources(const DComp &dc, const std::vector<int32_t> &s
     return out;
 }
 
-int batched_component(bcx_ctx *ctx, bcx_dgraph *, DComp &dc, const Plan &plan, const bcx_opts *o, double *bc,
+int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o, double *bc,
                       bcx_stats *st) {
-    const std::vector<int32_t> srcs = batched_sources(dc, plan);
     const int64_t ntiles = ((int64_t)srcs.size() + BATCH_W - 1) / BATCH_W;
     const int want = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width)) : MAX_SLOTS;
     int rc = bs_ensure(ctx, dc.n, (int32_t)(ntiles * BATCH_W), want, false);

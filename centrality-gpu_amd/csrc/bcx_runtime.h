@@ -46,6 +46,8 @@ struct DComp {
     int64_t arcs = 0;
     int64_t offset = 0;  // into the concatenated BC vector
     int32_t depth = -1;  // BFS depth from the highest-degree vertex (-1: not measured)
+    std::vector<int32_t> rank;      // batched path: BFS rank of each vertex (built on first use)
+    std::vector<uint8_t> on_chain;  // batched path: 1 for chain vertices (built on first use)
     HostComp host;
     DBuf b_rp, b_col, b_reach, b_chp, b_chn, b_che;
     CompDev dev{};
@@ -117,7 +119,7 @@ struct SmallJob {
     const bcx::DComp *dc;
     std::vector<int32_t> srcs;
 };
-std::vector<int32_t> batched_sources(const bcx::DComp &dc, const bcx::Plan &plan);
+std::vector<int32_t> batched_sources(bcx::DComp &dc, const int32_t *subset, int64_t n_sub, int64_t *n_chain);
 std::vector<int32_t> ball_sources(const bcx::DComp &dc, const std::vector<int32_t> &srcs);
 int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
                   bcx_stats *st);
@@ -125,8 +127,8 @@ int batched_wg(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *
                bcx_stats *st);
 bool batched_narrow(bcx::DComp &dc);
 void small_release(bcx_ctx *ctx);
-int batched_component(bcx_ctx *ctx, bcx_dgraph *g, bcx::DComp &dc, const bcx::Plan &plan,
-                      const bcx_opts *o, double *bc, bcx_stats *st);
+int batched_component(bcx_ctx *ctx, bcx::DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o,
+                      double *bc, bcx_stats *st);
 int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o,
                          int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta,
                          int32_t *sigma_is_f64);

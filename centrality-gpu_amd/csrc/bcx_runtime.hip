@@ -581,22 +581,34 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         const int32_t *sp = o.src_subset ? sub[ci].data() : nullptr;
         int64_t sn = o.src_subset ? (int64_t)sub[ci].size() : 0;
         if (o.src_subset && sn == 0) continue;
-        Plan plan;
-        rc = make_plan(dc.host, sp, sn, o.use_chains != 0, row_capacity(ctx, dc.n, &o), plan);
-        if (rc) return rc;
-        s->n_bfs_sources += plan.n_bfs;
-        s->n_chain_sources += plan.n_chain;
-        s->n_row_only += plan.n_row_only;
-        s->bytes_alg += (double)plan.n_bfs * (16.0 * dc.arcs + 80.0 * dc.n) +
-                        (double)plan.n_chain * (8.0 * dc.arcs + 100.0 * dc.n);
-        s->teps_work += (double)(plan.n_bfs + plan.n_chain) * (double)dc.arcs * 0.5;
-        if (o.path == BCX_PATH_PLAIN) rc = plain_component(ctx, dc, plan, &o, bc + dc.offset, s);
-        else if (dc.n <= SMALL_N && !(o.flags & BCX_FLAG_NO_SMALL_KERNEL))
-            small.push_back({&dc, batched_sources(dc, plan)});
+        // algorithmic work of the reference's method: BFS sources (free and
+        // chain endpoints) and chain sources
+        auto account = [&](int64_t n_bfs, int64_t n_chain) {
+            s->n_bfs_sources += n_bfs;
+            s->n_chain_sources += n_chain;
+            s->bytes_alg += (double)n_bfs * (16.0 * dc.arcs + 80.0 * dc.n) +
+                            (double)n_chain * (8.0 * dc.arcs + 100.0 * dc.n);
+            s->teps_work += (double)(n_bfs + n_chain) * (double)dc.arcs * 0.5;
+        };
+        if (o.path == BCX_PATH_PLAIN) {
+            Plan plan;
+            rc = make_plan(dc.host, sp, sn, o.use_chains != 0, row_capacity(ctx, dc.n, &o), plan);
+            if (rc) return rc;
+            account(plan.n_bfs, plan.n_chain);
+            s->n_row_only += plan.n_row_only;
+            rc = plain_component(ctx, dc, plan, &o, bc + dc.offset, s);
+            if (rc) return rc;
+            continue;
+        }
+        int64_t n_chain = 0;
+        std::vector<int32_t> srcs = batched_sources(dc, sp, sn, o.use_chains ? &n_chain : nullptr);
+        account((int64_t)srcs.size() - n_chain, n_chain);
+        if (dc.n <= SMALL_N && !(o.flags & BCX_FLAG_NO_SMALL_KERNEL))
+            small.push_back({&dc, std::move(srcs)});
         else if (dc.n <= WG_N && !(o.flags & BCX_FLAG_NO_WG_KERNEL) &&
                  ((o.flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
-            wg.push_back({&dc, ball_sources(dc, batched_sources(dc, plan))});
-        else rc = batched_component(ctx, g, dc, plan, &o, bc + dc.offset, s);
+            wg.push_back({&dc, ball_sources(dc, srcs)});
+        else rc = batched_component(ctx, dc, srcs, &o, bc + dc.offset, s);
         if (rc) return rc;
     }
     if (!small.empty() && (rc = batched_small(ctx, small, &o, bc, s))) return rc;
