@@ -95,11 +95,16 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "copies": 3}))
 
 
-def measured_traffic(slices_used, src_per_step):
+TRAFFIC_FILES = {2: "batched_config2_traffic.json", 4: "wg_config4_traffic.json"}
+
+
+def measured_traffic(config, src_per_step):
     """HBM bytes per step from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload (scaled per source when the slice differs)."""
-    p = os.path.join(ROOT, "profiles", "batched_config2_traffic.json")
-    if not os.path.exists(p):
+    summary of this workload (scaled per source when the slice differs);
+    None for workloads without one."""
+    name = TRAFFIC_FILES.get(config)
+    p = os.path.join(ROOT, "profiles", name) if name else None
+    if not p or not os.path.exists(p):
         return None
     with open(p) as f:
         t = json.load(f)
@@ -225,7 +230,7 @@ def main():
                 "sources": int(len(sample))}
 
     achieved = balg_b / tk_b / 1e9 if tk_b > 0 else None
-    traffic = measured_traffic(args.slices, src_b / K)
+    traffic = measured_traffic(args.config, src_b / K) if args.scale == 1.0 else None
     line = {
         "metric": "exact-BC MTEPS on 1 MI355X: batched FP32-comp vs plain, <=1e-6 rel err",
         "value": round(mteps_b, 3),
@@ -251,7 +256,7 @@ def main():
                      "traffic": round(traffic) if traffic else None,
                      "traffic_unit": "HBM bytes per step (rocprofv3 FETCH_SIZE+WRITE_SIZE, scaled)",
                      "algorithmic_bytes_per_step": round(balg_b / K),
-                     "kernel": "batched tile kernels (k_b_expand, k_b_pull, k_b_back) per step"},
+                     "kernel": "batched kernels of a step (k_b_expand, k_b_pull, k_b_back; k_small; k_wg)"},
         "cpu_baseline": cpu,
     }
     g.free()
