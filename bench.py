@@ -137,7 +137,7 @@ def main():
                     help="0 = per workload: 64 for the wide graphs, 1 for the deep grid (4)")
     ap.add_argument("--plain-slices", type=int, default=1024)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-sources", type=int, default=0, help="0 = 4 per host thread")
+    ap.add_argument("--cpu-sources", type=int, default=0, help="0 = 48 per host thread (10-30 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only a device copy (profiling)")
@@ -205,7 +205,7 @@ def main():
         sys.path.insert(0, ROOT)
         import oracle  # CPU baseline / checker only
         threads = oracle.num_threads()
-        k = args.cpu_sources or 4 * threads
+        k = args.cpu_sources or 48 * threads
         rng = np.random.default_rng(1234)
         sample = np.sort(rng.choice(c.n, size=min(k, c.n), replace=False)).astype(np.int32)
         tc = time.perf_counter()

@@ -52,6 +52,6 @@ hipError_t launch_b_forward(const BatchDev &b, int sm, int l, int grid, hipStrea
 hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st);
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st);
 hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st);
-hipError_t launch_b_fold(double *dst, double *src, int n, int grid, hipStream_t st);
+hipError_t launch_b_fold(double *dst, double *src, const int32_t *perm, int n, int grid, hipStream_t st);
 
 }  // namespace bcx

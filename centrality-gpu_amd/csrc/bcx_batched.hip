@@ -601,10 +601,11 @@ __global__ __launch_bounds__(BB) void k_b_clear_all(BatchDev b, int end) {
 
 // dst[v] += src[v]: folds a slot's BC accumulator into the component slice
 // and clears it for the next component.
-__global__ __launch_bounds__(BB) void k_b_fold(double *dst, double *src, int n) {
+// perm (optional) maps the tile's vertex ids to the component's.
+__global__ __launch_bounds__(BB) void k_b_fold(double *dst, double *src, const int32_t *perm, int n) {
     for (int v = blockIdx.x * BB + threadIdx.x; v < n; v += gridDim.x * BB) {
         const double x = src[v];
-        if (x != 0.0) dst[v] += x;
+        if (x != 0.0) dst[perm ? perm[v] : v] += x;
         src[v] = 0.0;
     }
 }
@@ -647,8 +648,8 @@ hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_b_fold(double *dst, double *src, int n, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_b_fold, dim3(grid), dim3(BB), 0, st, dst, src, n);
+hipError_t launch_b_fold(double *dst, double *src, const int32_t *perm, int n, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_b_fold, dim3(grid), dim3(BB), 0, st, dst, src, perm, n);
     return hipGetLastError();
 }
 

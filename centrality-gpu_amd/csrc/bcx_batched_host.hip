@@ -145,6 +145,7 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uin
 
 struct TileJob {
     const DComp *dc;
+    const int32_t *perm;  // tile vertex id -> component vertex id (nullptr: identity)
     double *bc;
     uint64_t limit;
     bool fp32;
@@ -283,7 +284,8 @@ static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob
     }
     // each slot accumulated its tiles' BC privately: fold into the component
     for (int i = 0; i < ns_slots && !rc; ++i)
-        rc = ctx_hip(ctx, launch_b_fold(j.bc, (double *)bs.slots[i].bcacc.p, dc.n, j.grid, ctx->stream), "fold");
+        rc = ctx_hip(ctx, launch_b_fold(j.bc, (double *)bs.slots[i].bcacc.p, j.perm, dc.n, j.grid, ctx->stream),
+                     "fold");
     ctx->timer.end(ctx);
     for (int i = 0; i < ns_slots; ++i) (void)hipStreamSynchronize(bs.slots[i].stream);
     (void)hipEventDestroy(e0);
@@ -403,6 +405,7 @@ bool batched_narrow(DComp &dc) {
 static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double *bc) {
     TileJob j;
     j.dc = &dc;
+    j.perm = nullptr;
     j.bc = bc;
     j.limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
                   ? (1ull << o->sigma_f64_threshold_log2) : 0;
@@ -474,14 +477,73 @@ std::vector<int32_t> ball_sources(const DComp &dc, const std::vector<int32_t> &s
     return out;
 }
 
+// The component relabelled by descending degree (stable): the hubs' rows,
+// masks and sigma / term rows, which most arcs point at, become one
+// contiguous, cache-resident block.  Each row keeps its arcs in the original
+// order, so the FP64 sigma sums keep the oracle's order.
+static int relabelled(bcx_ctx *ctx, DComp &dc, DComp *&out) {
+    if (dc.relabel) { out = dc.relabel; return BCX_OK; }
+    const HostComp &h = dc.host;
+    const int32_t n = h.n;
+    std::vector<int32_t> perm((size_t)n), inv((size_t)n);
+    for (int32_t v = 0; v < n; ++v) perm[v] = v;
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
+        return h.rp[a + 1] - h.rp[a] > h.rp[b + 1] - h.rp[b];
+    });
+    for (int32_t i = 0; i < n; ++i) inv[perm[i]] = i;
+    DComp *p = new DComp();
+    p->n = n;
+    p->arcs = dc.arcs;
+    p->offset = dc.offset;
+    HostComp &q = p->host;
+    q.n = n;
+    q.arcs = h.arcs;
+    q.rp.assign((size_t)n + 1, 0);
+    q.col.resize(h.col.size());
+    q.reach.resize((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t v = perm[i], a = h.rp[v], d = h.rp[v + 1] - a;
+        q.rp[i + 1] = q.rp[i] + d;
+        for (int32_t k = 0; k < d; ++k) q.col[q.rp[i] + k] = inv[h.col[a + k]];
+        q.reach[i] = h.reach[v];
+    }
+    q.ch_ptr.assign(1, 0);
+    int rc;
+    if ((rc = upload(ctx, q.rp.data(), 4 * q.rp.size(), p->b_rp, "relabelled component")) ||
+        (rc = upload(ctx, q.col.data(), 4 * q.col.size(), p->b_col, "relabelled component")) ||
+        (rc = upload(ctx, q.reach.data(), 4 * q.reach.size(), p->b_reach, "relabelled component")) ||
+        (rc = upload(ctx, perm.data(), 4 * perm.size(), dc.b_perm, "relabelling")) ||
+        (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "relabel"))) {
+        for (DBuf *b : {&p->b_rp, &p->b_col, &p->b_reach}) b->release(ctx);
+        delete p;
+        return rc;
+    }
+    p->dev.n = n;
+    p->dev.arcs = p->arcs;
+    p->dev.rp = (const int32_t *)p->b_rp.p;
+    p->dev.col = (const int32_t *)p->b_col.p;
+    p->dev.reach = (const int32_t *)p->b_reach.p;
+    dc.relabel_inv.swap(inv);
+    dc.relabel = p;
+    out = p;
+    return BCX_OK;
+}
+
 int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o, double *bc,
                       bcx_stats *st) {
-    const int64_t ntiles = ((int64_t)srcs.size() + BATCH_W - 1) / BATCH_W;
-    const int want = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width)) : MAX_SLOTS;
-    int rc = bs_ensure(ctx, dc.n, (int32_t)(ntiles * BATCH_W), want, false);
+    DComp *pc = nullptr;
+    int rc = relabelled(ctx, dc, pc);
     if (rc) return rc;
-    if ((rc = stage_sources(ctx, dc, srcs)) || (rc = stage_hubs(ctx, dc))) return rc;
-    return run_tiles(ctx, dc, (int64_t)srcs.size(), make_job(ctx, dc, o, bc), st, nullptr);
+    std::vector<int32_t> ps(srcs.size());
+    for (size_t i = 0; i < srcs.size(); ++i) ps[i] = dc.relabel_inv[srcs[i]];
+    const int64_t ntiles = ((int64_t)ps.size() + BATCH_W - 1) / BATCH_W;
+    const int want = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width)) : MAX_SLOTS;
+    rc = bs_ensure(ctx, pc->n, (int32_t)(ntiles * BATCH_W), want, false);
+    if (rc) return rc;
+    if ((rc = stage_sources(ctx, *pc, ps)) || (rc = stage_hubs(ctx, *pc))) return rc;
+    TileJob j = make_job(ctx, *pc, o, bc);
+    j.perm = (const int32_t *)dc.b_perm.p;
+    return run_tiles(ctx, *pc, (int64_t)ps.size(), j, st, nullptr);
 }
 
 // ------------------------------------------------------------------------
@@ -752,4 +814,13 @@ void batched_release(bcx_ctx *ctx) {
     ctx->batched = nullptr;
 }
 
-void batched_graph_release(bcx_ctx *, bcx_dgraph *) {}
+void batched_graph_release(bcx_ctx *ctx, bcx_dgraph *g) {
+    for (DComp &dc : g->comps) {
+        if (dc.relabel) {
+            for (DBuf *b : {&dc.relabel->b_rp, &dc.relabel->b_col, &dc.relabel->b_reach}) b->release(ctx);
+            delete dc.relabel;
+            dc.relabel = nullptr;
+        }
+        dc.b_perm.release(ctx);
+    }
+}
