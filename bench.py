@@ -96,6 +96,14 @@ def copy_probe(mib):
 
 
 TRAFFIC_FILES = {2: "batched_config2_traffic.json", 4: "wg_config4_traffic.json"}
+WORKLOADS = {
+    1: "16 Erdos-Renyi blocks of 128 vertices on a tree of cut vertices, 20% of edges as chains, "
+       "512 pendant vertices (10k vertices)",
+    2: "1M-vertex Chung-Lu core + degree-2 chains + pendant trees",
+    3: "519k-vertex Chung-Lu block + 40,000 small Erdos-Renyi blocks on a tree of cut vertices (12.8M vertices)",
+    4: "256x256 grid with 5% of edges deleted and 2,000 spliced chains (74k vertices)",
+    5: "R-MAT scale 20, edge factor 16, 10% of vertices as chains (750k vertices)",
+}
 
 
 def measured_traffic(config, src_per_step):
@@ -243,8 +251,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32 levels, u64/f64 sigma, fp32-kahan delta, f64 bc",
-        "data": "synthetic (seeded generator, config 2)",
-        "config": {"workload": "1M-vertex Chung-Lu core + degree-2 chains + pendant trees (config 2)",
+        "data": f"synthetic (seeded generator, workload {args.config}, scale {args.scale})",
+        "config": {"workload": WORKLOADS.get(args.config, f"workload {args.config}"), "generator": args.config,
                    "n": n, "m": m, "sources": total_src, "slices": args.slices,
                    "slice_sources_timed": src_b, "gpu": arch, "cus": n_cu},
         "plain": plain,

@@ -25,11 +25,13 @@
 // rho = delta / sigma, so the successor-method term
 // sigma_v/sigma_w (r(w)+1 + delta_w) (Codes/kernels_free.cu:378-421) becomes a
 // plain sum of one gathered word per (arc, lane); FP32 with Kahan
-// compensation (or FP64).  BC(v) += sum_lanes (r(s)+1) sigma_v rho_v
-// (Codes/kernels_free.cu:423-426), one FP64 atomic per (vertex, level).
+// compensation (FP64 in FP64 precision and in FP64-sigma tiles).
+// BC(v) += sum_lanes (r(s)+1) sigma_v rho_v (Codes/kernels_free.cu:423-426),
+// into the slot's own FP64 accumulator (a vertex is finished once per level
+// launch, so no atomics), folded into the component at the end of a run.
 //
 // Hubs (degree > hub_deg) are cut into static arc slices shared by several
-// waves: uint64 sigma partials are added with 64-bit atomics (exact), rho
+// waves: integer sigma partials are added with atomics (exact), rho
 // partials into FP64 accumulators and the last slice to finish a hub
 // finalises it.  FP64-sigma tiles keep hubs whole in the pull (fixed order).
 //

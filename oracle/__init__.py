@@ -1,7 +1,7 @@
 """ctypes loader of the CPU oracle (oracle/bcx_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, always as the checker, never as the product.
+CPU-baseline leg of bench.py, always as the reference result, never as the product.
 See the header of bcx_oracle.c for what each function restates and how it is
 pinned.
 """

@@ -3,7 +3,7 @@
 Bar: levels and sigma bit-exact against the oracle (uint64, and FP64 sigma in
 the oracle's CSR pull order); with FP64 dependency terms delta within 1e-12
 and BC within 1e-9 relative; with FP32-compensated terms delta within 2e-6
-and BC within 1e-6 max relative; zero BC exact.
+and BC within 1e-6 max relative (the FP32C bar); zero BC exact.
 """
 import numpy as np
 import pytest
