@@ -192,3 +192,74 @@ def test_cli_batched_fp32(goldens, tmp_path):
     assert r.returncode == 0, r.stderr
     mx, _, zeros = relerr(np.loadtxt(out)[:, 1], goldens["sample_test4"]["bc"])
     assert mx < 1e-6 and zeros
+
+
+def _write_edges(path, n, uv):
+    with open(path, "w") as f:
+        f.write(f"{n} {len(uv)}\n")
+        for a, b in uv:
+            f.write(f"{a} {b} 1\n")
+
+
+def _structure(stdout):
+    """stdout with every number after a time label blanked (timings differ)."""
+    import re
+    out = []
+    for l in stdout.splitlines():
+        if "time" in l:
+            l = re.sub(r"[-+0-9.eE]+\s*$", "T", l)
+        out.append(l)
+    return out
+
+
+def test_cli_batched_many_components(tmp_path):
+    """The batched CLI uploads every component once and runs them in one
+    call: same stdout structure as the per-component plain run (timing values
+    aside) and whole-graph BC equal to the oracle's."""
+    n, uv = bcx.generate(3, 0, 0.001)
+    uv = [tuple(map(int, e)) for e in np.asarray(uv).reshape(-1, 2)]
+    src = tmp_path / "g.txt"
+    _write_edges(src, n, uv)
+    exe = os.path.join(bcx.HERE, "betc")
+    rp, col = oracle.csr_from_edges(n, uv)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    outs = {}
+    for name, extra, tol in (("plain", [], 1e-9), ("batched", ["--batched", "--fp32"], 1e-6)):
+        out = tmp_path / f"bc_{name}.txt"
+        r = subprocess.run([exe, str(src), "--bc-out-exact", str(out)] + extra, capture_output=True, text=True,
+                           cwd=tmp_path, timeout=300)
+        assert r.returncode == 0, r.stderr
+        mx, _, zeros = relerr(np.loadtxt(out)[:, 1], want)
+        print(name, "maxrel", mx, flush=True)
+        assert mx < tol and zeros
+        outs[name] = _structure(r.stdout)
+    assert sum(l.startswith("BCC number #: ") for l in outs["plain"]) > 20
+    assert outs["plain"] == outs["batched"]
+
+
+def test_cli_sampled(tmp_path):
+    """--sample K: exact when K covers every component; with K below the
+    giant component's size, the cut terms stay exact and the sampled,
+    rescaled dependency sums stay close to the exact ones."""
+    n, uv = bcx.generate(2, 0, 0.003)
+    uv = [tuple(map(int, e)) for e in np.asarray(uv).reshape(-1, 2)]
+    src = tmp_path / "g.txt"
+    _write_edges(src, n, uv)
+    exe = os.path.join(bcx.HERE, "betc")
+    rp, col = oracle.csr_from_edges(n, uv)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    h = bcx.Host.from_edges(n, uv).decompose()
+    big = max(c.n for c in h.components())
+    out = tmp_path / "bc.txt"
+    r = subprocess.run([exe, str(src), "--batched", "--sample", str(big), "--bc-out-exact", str(out)],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stderr
+    mx, _, zeros = relerr(np.loadtxt(out)[:, 1], want)
+    assert mx < 1e-9 and zeros
+    r = subprocess.run([exe, str(src), "--sample", str(big // 3), "--seed", "7", "--bc-out-exact", str(out)],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(out)[:, 1]
+    assert abs(got.sum() - want.sum()) / want.sum() < 0.05
+    top = np.argsort(-want)[:20]
+    assert np.all(np.abs(got[top] - want[top]) / want[top] < 0.5)
