@@ -177,7 +177,9 @@ def main():
 
     eng = bcx.Engine(0)
     n_cu, arch = eng.info()
+    t0 = time.perf_counter()
     g = eng.upload(comps)
+    print(f"# upload of {len(comps)} components {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
     W = float(n) * float(m)
 
     # batched FP32-compensated path (the headline)

@@ -106,6 +106,7 @@ struct bcx_ctx {
 struct bcx_dgraph {
     std::vector<bcx::DComp> comps;
     int64_t total_n = 0;
+    bcx::DBuf pool;  // every component's CSR, reach and chain arrays, one allocation
 };
 
 // shared helpers (bcx_runtime.hip)

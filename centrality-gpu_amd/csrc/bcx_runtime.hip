@@ -506,27 +506,46 @@ int bcx_upload(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps, bcx_dg
         } else {
             hc.ch_ptr.assign(1, 0);
         }
-        int rc;
-        if ((rc = upload(ctx, hc.rp.data(), hc.rp.size() * 4, dc.b_rp, "row_ptr")) ||
-            (rc = upload(ctx, hc.col.data(), hc.col.size() * 4, dc.b_col, "col")) ||
-            (rc = upload(ctx, hc.reach.data(), hc.reach.size() * 4, dc.b_reach, "reach")) ||
-            (rc = upload(ctx, hc.ch_ptr.data(), hc.ch_ptr.size() * 4, dc.b_chp, "chain_ptr")) ||
-            (rc = upload(ctx, hc.ch_nodes.data(), hc.ch_nodes.size() * 4, dc.b_chn, "chain_nodes")) ||
-            (rc = upload(ctx, hc.ch_ends.data(), hc.ch_ends.size() * 4, dc.b_che, "chain_ends"))) {
-            bcx_dgraph_free(ctx, g);
-            return rc;
-        }
-        dc.dev.n = dc.n;
-        dc.dev.arcs = dc.arcs;
-        dc.dev.rp = (const int32_t *)dc.b_rp.p;
-        dc.dev.col = (const int32_t *)dc.b_col.p;
-        dc.dev.reach = (const int32_t *)dc.b_reach.p;
-        dc.chains.ptr = (const int32_t *)dc.b_chp.p;
-        dc.chains.nodes = (const int32_t *)dc.b_chn.p;
-        dc.chains.ends = (const int32_t *)dc.b_che.p;
     }
     g->total_n = off;
-    int rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "upload");
+    // every array of every component in one device allocation and one copy
+    // (tens of thousands of small components would otherwise cost an
+    // allocation and a copy per array); arrays start on 256-byte boundaries
+    auto padded = [](size_t k) { return (k + 63) & ~(size_t)63; };
+    size_t words = 0;
+    for (DComp &dc : g->comps) {
+        const HostComp &hc = dc.host;
+        words += padded(hc.rp.size()) + padded(hc.col.size()) + padded(hc.reach.size()) + padded(hc.ch_ptr.size()) +
+                 padded(hc.ch_nodes.size()) + padded(hc.ch_ends.size());
+    }
+    int rc = g->pool.ensure(ctx, 4 * std::max<size_t>(words, 1), "components");
+    if (rc) { bcx_dgraph_free(ctx, g); return rc; }
+    std::vector<int32_t> stage(words, 0);
+    int32_t *base = (int32_t *)g->pool.p;
+    size_t at = 0;
+    auto place = [&](const std::vector<int32_t> &v) -> const int32_t * {
+        std::copy(v.begin(), v.end(), stage.begin() + at);
+        const int32_t *p = base + at;
+        at += padded(v.size());
+        return p;
+    };
+    for (DComp &dc : g->comps) {
+        const HostComp &hc = dc.host;
+        dc.dev.n = dc.n;
+        dc.dev.arcs = dc.arcs;
+        dc.dev.rp = place(hc.rp);
+        dc.dev.col = place(hc.col);
+        dc.dev.reach = place(hc.reach);
+        dc.chains.ptr = place(hc.ch_ptr);
+        dc.chains.nodes = place(hc.ch_nodes);
+        dc.chains.ends = place(hc.ch_ends);
+    }
+    if (words && (rc = ctx_hip(ctx, hipMemcpyAsync(g->pool.p, stage.data(), 4 * words, hipMemcpyHostToDevice,
+                                                   ctx->stream), "components"))) {
+        bcx_dgraph_free(ctx, g);
+        return rc;
+    }
+    rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "upload");
     if (rc) { bcx_dgraph_free(ctx, g); return rc; }
     *out = g;
     return BCX_OK;
@@ -539,6 +558,7 @@ void bcx_dgraph_free(bcx_ctx *ctx, bcx_dgraph *g) {
         for (auto &dc : g->comps)
             for (DBuf *b : {&dc.b_rp, &dc.b_col, &dc.b_reach, &dc.b_chp, &dc.b_chn, &dc.b_che}) b->release(ctx);
         batched_graph_release(ctx, g);
+        g->pool.release(ctx);
     }
     delete g;
 }
