@@ -384,20 +384,28 @@ bool batched_narrow(DComp &dc) {
         for (int32_t v = 1; v < hc.n; ++v)
             if (hc.rp[v + 1] - hc.rp[v] > hc.rp[root + 1] - hc.rp[root]) root = v;
         std::vector<int32_t> lv((size_t)hc.n, -1), q;
+        std::vector<double> sg((size_t)hc.n, 0.0);
         q.reserve(hc.n);
         q.push_back(root);
         lv[root] = 0;
+        sg[root] = 1.0;
         int32_t depth = 0;
+        double smax = 1.0;
         for (size_t i = 0; i < q.size(); ++i) {
             const int32_t v = q[i];
             depth = std::max(depth, lv[v]);
-            for (int32_t j = hc.rp[v]; j < hc.rp[v + 1]; ++j)
-                if (lv[hc.col[j]] < 0) {
-                    lv[hc.col[j]] = lv[v] + 1;
-                    q.push_back(hc.col[j]);
+            smax = std::max(smax, sg[v]);
+            for (int32_t j = hc.rp[v]; j < hc.rp[v + 1]; ++j) {
+                const int32_t w = hc.col[j];
+                if (lv[w] < 0) {
+                    lv[w] = lv[v] + 1;
+                    q.push_back(w);
                 }
+                if (lv[w] == lv[v] + 1) sg[w] += sg[v];
+            }
         }
         dc.depth = depth;
+        dc.sigma_est = smax;
     }
     return (int64_t)dc.n <= (int64_t)WG_WIDTH * (dc.depth + 1);
 }
@@ -610,8 +618,18 @@ static int persistent_tiles(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     std::vector<SmallTile> tiles;
     std::vector<int32_t> srcs;
     std::vector<double> ws;
-    for (const SmallJob &jb : jobs) {
+    // workgroup kernel: components whose path counts from their highest-degree
+    // vertex already pass 2^62 start with FP64 sigma (their tiles would all
+    // overflow the 64-bit pass); their tiles go after the others
+    std::vector<const SmallJob *> order;
+    for (const SmallJob &jb : jobs) order.push_back(&jb);
+    auto wide = [&](const SmallJob *jb) { return wg && jb->dc->sigma_est > 4.6e18; };
+    std::stable_partition(order.begin(), order.end(), [&](const SmallJob *jb) { return !wide(jb); });
+    size_t n_narrow_tiles = 0;
+    for (const SmallJob *jbp : order) {
+        const SmallJob &jb = *jbp;
         const DComp &dc = *jb.dc;
+        if (!wide(jbp)) n_narrow_tiles += (jb.srcs.size() + BATCH_W - 1) / BATCH_W;
         SmallComp c{};
         c.n = dc.n;
         c.rp = dc.dev.rp;
@@ -686,23 +704,31 @@ static int persistent_tiles(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
         int r = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "workgroup reset");
         return r ? r : ctx_hip(ctx, launch_wg(a, f64, fp32, (int)grid, ctx->stream), "workgroup kernel");
     };
-    ctx->timer.begin(ctx);
-    if ((rc = launch(false, use_graph))) return rc;
-    ctx->timer.end(ctx);
-    if (st) st->n_launches += 2;
-    int32_t h[2];
-    if ((rc = ctx_hip(ctx, hipMemcpyAsync(h, a.counter, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "counter")) ||
-        (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel")))
-        return rc;
-    if (h[1] > 0) {
-        // tiles whose uint64 sigma overflowed added nothing: rerun them in FP64
+    // uint64 pass over the tiles not predicted to overflow
+    int32_t h[2] = {0, 0};
+    a.n_tiles = (int32_t)n_narrow_tiles;
+    if (a.n_tiles > 0) {
+        ctx->timer.begin(ctx);
+        if ((rc = launch(false, use_graph))) return rc;
+        ctx->timer.end(ctx);
+        if (st) st->n_launches += 2;
+        if ((rc = ctx_hip(ctx, hipMemcpyAsync(h, a.counter, sizeof(h), hipMemcpyDeviceToHost, ctx->stream),
+                          "counter")) ||
+            (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel")))
+            return rc;
+    }
+    if (h[1] > 0 || n_narrow_tiles < tiles.size()) {
+        // tiles whose uint64 sigma overflowed added nothing: rerun them in
+        // FP64, with the tiles that start there
         std::vector<int32_t> redo((size_t)h[1]);
-        if ((rc = ctx_hip(ctx, hipMemcpyAsync(redo.data(), a.redo, 4 * redo.size(), hipMemcpyDeviceToHost,
-                                              ctx->stream), "redo")) ||
-            (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "redo")))
+        if (h[1] > 0 &&
+            ((rc = ctx_hip(ctx, hipMemcpyAsync(redo.data(), a.redo, 4 * redo.size(), hipMemcpyDeviceToHost,
+                                               ctx->stream), "redo")) ||
+             (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "redo"))))
             return rc;
         std::vector<SmallTile> again;
         for (int32_t t : redo) again.push_back(tiles[t]);
+        for (size_t t = n_narrow_tiles; t < tiles.size(); ++t) again.push_back(tiles[t]);
         std::sort(again.begin(), again.end(), [](const SmallTile &x, const SmallTile &y) { return x.src0 < y.src0; });
         if ((rc = upload(ctx, again.data(), sizeof(SmallTile) * again.size(), ss.tiles, "small tiles"))) return rc;
         a.tiles = (const SmallTile *)ss.tiles.p;

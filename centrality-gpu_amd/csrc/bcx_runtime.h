@@ -46,6 +46,7 @@ struct DComp {
     int64_t arcs = 0;
     int64_t offset = 0;  // into the concatenated BC vector
     int32_t depth = -1;  // BFS depth from the highest-degree vertex (-1: not measured)
+    double sigma_est = 0.0;  // largest path count of that BFS (measured with depth)
     std::vector<int32_t> rank;      // batched path: BFS rank of each vertex (built on first use)
     std::vector<uint8_t> on_chain;  // batched path: 1 for chain vertices (built on first use)
     HostComp host;
