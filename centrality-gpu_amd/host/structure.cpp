@@ -8,16 +8,62 @@
 //   * degree-2 chains (ear pieces)            Codes/bcc_bfs.cpp:100-236, 928-1311
 //
 // Everything is iterative and linear in n + m; the reference's recursive DFS
-// (Codes/bicc.cpp:81) overflows the stack at a million vertices.
+// (Codes/bicc.cpp:81) overflows the stack at a million vertices.  The passes
+// that are independent per edge, vertex or component run on host threads
+// (BCX_HOST_THREADS, else OMP_NUM_THREADS, else the hardware count); the DFS
+// is sequential and keeps the reference's visit order.
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "host_graph.h"
 
 namespace bcx {
+
+static int host_threads() {
+    for (const char *k : {"BCX_HOST_THREADS", "OMP_NUM_THREADS"}) {
+        const char *e = getenv(k);
+        if (e && atoi(e) > 0) return std::min(atoi(e), 64);
+    }
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(hc, 64u));
+}
+
+// f(t, begin, end) over T contiguous ranges of [0, n); inline when small.
+template <class F>
+static void parallel_ranges(int64_t n, int T, F f) {
+    if (T <= 1 || n < (1 << 16)) {
+        f(0, (int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t] { f(t, n * t / T, n * (t + 1) / T); });
+    for (auto &x : th) x.join();
+}
+
+// f(t) on T threads (inline when T is 1).
+template <class F>
+static void parallel_tasks(int T, F f) {
+    if (T <= 1) {
+        f(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (int t = 0; t < T; ++t) th.emplace_back([&, t] { f(t); });
+    for (auto &x : th) x.join();
+}
 
 bcx_component Component::view() const {
     bcx_component c{};
@@ -33,66 +79,116 @@ bcx_component Component::view() const {
     return c;
 }
 
+// Adjacency in file order (Codes/graph.cpp:52-57: adj[u] gets v, then adj[v]
+// gets u, edge by edge).  The edges are cut into T contiguous ranges; each
+// range counts its arcs per vertex, the counts give every (range, vertex) its
+// first slot, and each range fills its slots in edge order, so the lists are
+// the sequential ones.
 int build_adjacency(bcx_host *h) {
     const int32_t n = h->n;
     const int64_t m = h->m;
-    h->adj_ptr.assign((size_t)n + 1, 0);
-    for (int64_t e = 0; e < m; ++e) {
-        int32_t u = h->uv[2 * e], v = h->uv[2 * e + 1];
-        if (u < 0 || u >= n || v < 0 || v >= n) {
-            h->err = "edge " + std::to_string(e) + " has a vertex id outside [0, n)";
-            return BCX_E_IO;
+    const int32_t *uv = h->uv.data();
+    // per-range counters cost 4 bytes per (range, vertex): at most ~1 GiB
+    int T = host_threads();
+    while (T > 1 && (double)T * n * 4.0 > (double)(1ll << 30)) --T;
+    if (m < (1 << 16)) T = 1;
+    std::atomic<int64_t> bad(INT64_MAX);
+    std::vector<std::vector<int32_t>> cnt((size_t)T);
+    parallel_ranges(m, T, [&](int t, int64_t e0, int64_t e1) {
+        std::vector<int32_t> &c = cnt[t];
+        c.assign((size_t)n, 0);
+        for (int64_t e = e0; e < e1; ++e) {
+            const int32_t u = uv[2 * e], v = uv[2 * e + 1];
+            if ((uint32_t)u >= (uint32_t)n || (uint32_t)v >= (uint32_t)n) {
+                int64_t cur = bad.load();
+                while (e < cur && !bad.compare_exchange_weak(cur, e)) {}
+                return;
+            }
+            c[u]++;
+            c[v]++;
         }
-        h->adj_ptr[u + 1]++;
-        h->adj_ptr[v + 1]++;
+    });
+    if (bad.load() != INT64_MAX) {
+        h->err = "edge " + std::to_string(bad.load()) + " has a vertex id outside [0, n)";
+        return BCX_E_IO;
     }
-    for (int32_t i = 0; i < n; ++i) h->adj_ptr[i + 1] += h->adj_ptr[i];
+    h->adj_ptr.assign((size_t)n + 1, 0);
+    int64_t *ap = h->adj_ptr.data();
+    parallel_ranges(n, T, [&](int, int64_t v0, int64_t v1) {
+        for (int64_t v = v0; v < v1; ++v) {
+            int64_t d = 0;
+            for (int t = 0; t < T; ++t) d += cnt[t][v];
+            ap[v + 1] = d;
+        }
+    });
+    for (int32_t i = 0; i < n; ++i) ap[i + 1] += ap[i];
+    // cnt[t][v] becomes range t's first slot in v's list (fits in int32: it
+    // is an offset inside the list)
+    parallel_ranges(n, T, [&](int, int64_t v0, int64_t v1) {
+        for (int64_t v = v0; v < v1; ++v) {
+            int32_t run = 0;
+            for (int t = 0; t < T; ++t) {
+                const int32_t c = cnt[t][v];
+                cnt[t][v] = run;
+                run += c;
+            }
+        }
+    });
     h->adj.assign((size_t)(2 * m), 0);
-    std::vector<int64_t> fill(h->adj_ptr.begin(), h->adj_ptr.end() - 1);
-    // Codes/graph.cpp:52-57: adj[u] gets v, then adj[v] gets u, in file order.
-    for (int64_t e = 0; e < m; ++e) {
-        int32_t u = h->uv[2 * e], v = h->uv[2 * e + 1];
-        h->adj[fill[u]++] = v;
-        h->adj[fill[v]++] = u;
-    }
+    int32_t *adj = h->adj.data();
+    parallel_ranges(m, T, [&](int t, int64_t e0, int64_t e1) {
+        std::vector<int32_t> &c = cnt[t];
+        for (int64_t e = e0; e < e1; ++e) {
+            const int32_t u = uv[2 * e], v = uv[2 * e + 1];
+            adj[ap[u] + c[u]++] = v;
+            adj[ap[v] + c[v]++] = u;
+        }
+    });
     return BCX_OK;
 }
 
-static int check_simple_connected(bcx_host *h) {
+// Self loops (first by edge order) and duplicate edges (first vertex u whose
+// list repeats a neighbour, reported at the repeat).  Connectivity is checked
+// from the DFS of vertex 0 in tarjan().
+static int check_simple(bcx_host *h) {
     const int32_t n = h->n;
-    for (int64_t e = 0; e < h->m; ++e)
-        if (h->uv[2 * e] == h->uv[2 * e + 1]) {
-            h->err = "self loop on vertex " + std::to_string(h->uv[2 * e]);
-            return BCX_E_GRAPH;
+    const int T = host_threads();
+    std::atomic<int64_t> loop(INT64_MAX), dup(INT64_MAX);
+    auto lower = [](std::atomic<int64_t> &a, int64_t x) {
+        int64_t cur = a.load();
+        while (x < cur && !a.compare_exchange_weak(cur, x)) {}
+    };
+    parallel_ranges(h->m, T, [&](int, int64_t e0, int64_t e1) {
+        for (int64_t e = e0; e < e1; ++e)
+            if (h->uv[2 * e] == h->uv[2 * e + 1]) { lower(loop, e); return; }
+    });
+    if (loop.load() != INT64_MAX) {
+        h->err = "self loop on vertex " + std::to_string(h->uv[2 * loop.load()]);
+        return BCX_E_GRAPH;
+    }
+    const int64_t *ap = h->adj_ptr.data();
+    const int32_t *adj = h->adj.data();
+    parallel_ranges(n, T, [&](int, int64_t u0, int64_t u1) {
+        std::vector<int32_t> tmp;
+        for (int64_t u = u0; u < u1 && u < dup.load(std::memory_order_relaxed); ++u) {
+            const int64_t d = ap[u + 1] - ap[u];
+            if (d < 2) continue;
+            tmp.assign(adj + ap[u], adj + ap[u + 1]);
+            std::sort(tmp.begin(), tmp.end());
+            if (std::adjacent_find(tmp.begin(), tmp.end()) != tmp.end()) { lower(dup, u); return; }
         }
-    std::vector<int32_t> mark((size_t)n, -1);
-    for (int32_t u = 0; u < n; ++u)
-        for (int64_t j = h->adj_ptr[u]; j < h->adj_ptr[u + 1]; ++j) {
-            int32_t v = h->adj[j];
-            if (mark[v] == u) {
+    });
+    if (dup.load() != INT64_MAX) {
+        const int32_t u = (int32_t)dup.load();
+        std::vector<int32_t> seen;
+        for (int64_t j = ap[u]; j < ap[u + 1]; ++j) {
+            const int32_t v = adj[j];
+            if (std::find(seen.begin(), seen.end(), v) != seen.end()) {
                 h->err = "duplicate edge " + std::to_string(u) + " " + std::to_string(v);
                 return BCX_E_GRAPH;
             }
-            mark[v] = u;
+            seen.push_back(v);
         }
-    if (n == 0) return BCX_OK;
-    std::vector<int32_t> q;
-    q.reserve(n);
-    std::vector<uint8_t> seen((size_t)n, 0);
-    seen[0] = 1;
-    q.push_back(0);
-    for (size_t i = 0; i < q.size(); ++i) {
-        int32_t u = q[i];
-        for (int64_t j = h->adj_ptr[u]; j < h->adj_ptr[u + 1]; ++j)
-            if (!seen[h->adj[j]]) {
-                seen[h->adj[j]] = 1;
-                q.push_back(h->adj[j]);
-            }
-    }
-    if ((int32_t)q.size() != n) {
-        h->err = "graph is disconnected (" + std::to_string(q.size()) + " of " +
-                 std::to_string(n) + " vertices reachable from 0)";
-        return BCX_E_GRAPH;
     }
     return BCX_OK;
 }
@@ -101,14 +197,24 @@ static int check_simple_connected(bcx_host *h) {
 // in file order, tree edge pushed before descending, back edge pushed when
 // dfs[v] < dfs[u] and v is not the DFS parent, a component popped when
 // dfs[u] <= low[v] after returning from v (Codes/bicc.cpp:81-161).
-static void tarjan(bcx_host *h) {
+// The per-vertex DFS state is one 32-byte record (one cache line touched per
+// visit) and the records of the next neighbours in a list are prefetched:
+// the walk is bound by those random reads.  Returns the number of vertices
+// reached from vertex 0.
+struct DfsRec {
+    int32_t dfs, low, parent, pad;
+    int64_t it, end;  // next arc to scan, end of the list
+};
+static int32_t tarjan(bcx_host *h) {
     const int32_t n = h->n;
-    std::vector<int32_t> dfs((size_t)n, -1), low((size_t)n, -1);
-    std::vector<int64_t> it((size_t)n, 0);
-    std::vector<int32_t> parent((size_t)n, -1);
+    constexpr int PF = 6;  // neighbours prefetched ahead of the scan
+    std::vector<DfsRec> rec((size_t)n);
+    DfsRec *R = rec.data();
+    const int32_t *adj = h->adj.data();
+    for (int32_t v = 0; v < n; ++v) R[v] = DfsRec{-1, -1, -1, 0, h->adj_ptr[v], h->adj_ptr[v + 1]};
     std::vector<int32_t> stk;            // DFS vertex stack
     std::vector<int32_t> estk;           // edge stack (u,v pairs)
-    int32_t last = 0;
+    int32_t last = 0, reached0 = 0;
     int32_t children_of_0 = 0;           // Codes/bicc.cpp:13,86-87
     h->is_art.assign((size_t)n, 0);
     h->bcc_ptr.assign(1, 0);
@@ -117,37 +223,41 @@ static void tarjan(bcx_host *h) {
     std::vector<int32_t> vmark((size_t)n, -1);
 
     for (int32_t r = 0; r < n; ++r) {
-        if (dfs[r] != -1) continue;
-        dfs[r] = low[r] = ++last;
-        parent[r] = -1;
-        it[r] = h->adj_ptr[r];
+        if (R[r].dfs != -1) continue;
+        R[r].dfs = R[r].low = ++last;
+        R[r].parent = -1;
         stk.push_back(r);
+        for (int64_t j = R[r].it; j < std::min(R[r].end, R[r].it + PF); ++j) __builtin_prefetch(&R[adj[j]]);
         while (!stk.empty()) {
-            int32_t u = stk.back();
-            if (it[u] < h->adj_ptr[u + 1]) {
-                int32_t v = h->adj[it[u]++];
-                if (dfs[v] == -1) {
+            const int32_t u = stk.back();
+            DfsRec &U = R[u];
+            if (U.it < U.end) {
+                if (U.it + PF < U.end) __builtin_prefetch(&R[adj[U.it + PF]]);
+                const int32_t v = adj[U.it++];
+                DfsRec &V = R[v];
+                if (V.dfs == -1) {
                     estk.push_back(u);
                     estk.push_back(v);
-                    dfs[v] = low[v] = ++last;
-                    parent[v] = u;
+                    V.dfs = V.low = ++last;
+                    V.parent = u;
                     if (u == 0) children_of_0++;
-                    it[v] = h->adj_ptr[v];
                     stk.push_back(v);
-                } else if (dfs[v] < dfs[u] && v != parent[u]) {
+                    for (int64_t j = V.it; j < std::min(V.end, V.it + PF); ++j) __builtin_prefetch(&R[adj[j]]);
+                } else if (V.dfs < U.dfs && v != U.parent) {
                     estk.push_back(u);
                     estk.push_back(v);
-                    if (dfs[v] < low[u]) low[u] = dfs[v];
+                    if (V.dfs < U.low) U.low = V.dfs;
                 }
                 continue;
             }
             // u finished: return to its parent p with child u
             stk.pop_back();
-            int32_t p = parent[u];
+            const int32_t p = U.parent;
             if (p < 0) continue;
-            int32_t c = u;
-            if (low[c] < low[p]) low[p] = low[c];
-            if (dfs[p] <= low[c] && !estk.empty()) {
+            const int32_t c = u;
+            DfsRec &P = R[p];
+            if (U.low < P.low) P.low = U.low;
+            if (P.dfs <= U.low && !estk.empty()) {
                 int32_t bcc_id = (int32_t)h->bcc_nv.size();
                 int32_t nv = 0;
                 while (!estk.empty()) {
@@ -164,7 +274,9 @@ static void tarjan(bcx_host *h) {
                 if (p != 0 || children_of_0 > 1) h->is_art[p] = 1;
             }
         }
+        if (r == 0) reached0 = last;
     }
+    return reached0;
 }
 
 // Block-cut tree: nodes are BCCs [0, nb) and articulation vertices.  Each
@@ -293,62 +405,144 @@ static void extract_chains(Component &c) {
     }
 }
 
+// One component: first-appearance local ids over the BCC's edge list
+// (Codes/modified_apsp.cpp:336-357), arcs (u->v),(v->u) per edge in a stable
+// counting sort by source, reach of its articulation vertices, chains.
+// `local` is all -1 on entry and on return.
+static void build_component(const bcx_host *h, int32_t b, const std::vector<std::pair<int32_t, int32_t>> &reach,
+                            bool use_chains, std::vector<int32_t> &local, Component &c) {
+    c.bcc = b;
+    const int64_t e0 = h->bcc_ptr[b], e1 = h->bcc_ptr[b + 1];
+    const int64_t ne = e1 - e0;
+    const int32_t *buv = h->bcc_uv.data();
+    for (int64_t e = e0; e < e1; ++e)
+        for (int k = 0; k < 2; ++k) {
+            const int32_t v = buv[2 * e + k];
+            if (local[v] < 0) { local[v] = (int32_t)c.gid.size(); c.gid.push_back(v); }
+        }
+    c.n = (int32_t)c.gid.size();
+    c.row_ptr.assign((size_t)c.n + 1, 0);
+    for (int64_t e = e0; e < e1; ++e) {
+        c.row_ptr[local[buv[2 * e]] + 1]++;
+        c.row_ptr[local[buv[2 * e + 1]] + 1]++;
+    }
+    for (int32_t i = 0; i < c.n; ++i) c.row_ptr[i + 1] += c.row_ptr[i];
+    c.col.assign((size_t)(2 * ne), 0);
+    std::vector<int32_t> fill(c.row_ptr.begin(), c.row_ptr.end() - 1);
+    for (int64_t e = e0; e < e1; ++e) {
+        const int32_t u = local[buv[2 * e]], v = local[buv[2 * e + 1]];
+        c.col[fill[u]++] = v;
+        c.col[fill[v]++] = u;
+    }
+    c.reach.assign((size_t)c.n, 0);
+    for (auto &pr : reach) c.reach[local[pr.first]] = pr.second;
+    for (int32_t v : c.gid) local[v] = -1;
+    if (use_chains) {
+        extract_chains(c);
+    } else {
+        c.chain_ptr.assign(1, 0);
+        c.n_free = c.n;
+    }
+}
+
 int decompose(bcx_host *h, bool use_chains, bool check_input) {
     if (h->adj_ptr.size() != (size_t)h->n + 1) {
         int rc = build_adjacency(h);
         if (rc) return rc;
     }
     if (check_input) {
-        int rc = check_simple_connected(h);
+        int rc = check_simple(h);
         if (rc) return rc;
     }
-    tarjan(h);
+    const int32_t reached0 = tarjan(h);
+    if (check_input && h->n > 0 && reached0 != h->n) {
+        h->err = "graph is disconnected (" + std::to_string(reached0) + " of " +
+                 std::to_string(h->n) + " vertices reachable from 0)";
+        return BCX_E_GRAPH;
+    }
     std::vector<std::vector<std::pair<int32_t, int32_t>>> reach_lists;
     block_cut(h, reach_lists);
 
+    // components (BCCs of more than two vertices, Codes/to_eargraph_construct.cpp:59-60)
+    // are built independently; each thread keeps its own global -> local map
     const int32_t nb = (int32_t)h->bcc_nv.size();
+    std::vector<int32_t> blocks;
+    for (int32_t b = 0; b < nb; ++b)
+        if (h->bcc_nv[b] > 2) blocks.push_back(b);
     h->comps.clear();
-    std::vector<int32_t> local((size_t)h->n, -1);
-    for (int32_t b = 0; b < nb; ++b) {
-        if (h->bcc_nv[b] <= 2) continue;  // Codes/to_eargraph_construct.cpp:59-60
-        Component c;
-        c.bcc = b;
-        const int64_t e0 = h->bcc_ptr[b], e1 = h->bcc_ptr[b + 1];
-        const int64_t ne = e1 - e0;
-        // first-appearance local ids (Codes/modified_apsp.cpp:336-357)
-        for (int64_t e = e0; e < e1; ++e)
-            for (int k = 0; k < 2; ++k) {
-                int32_t v = h->bcc_uv[2 * e + k];
-                if (local[v] < 0) { local[v] = (int32_t)c.gid.size(); c.gid.push_back(v); }
-            }
-        c.n = (int32_t)c.gid.size();
-        // arcs (u->v),(v->u) per edge, stable counting sort by source
-        c.row_ptr.assign((size_t)c.n + 1, 0);
-        for (int64_t e = e0; e < e1; ++e) {
-            c.row_ptr[local[h->bcc_uv[2 * e]] + 1]++;
-            c.row_ptr[local[h->bcc_uv[2 * e + 1]] + 1]++;
-        }
-        for (int32_t i = 0; i < c.n; ++i) c.row_ptr[i + 1] += c.row_ptr[i];
-        c.col.assign((size_t)(2 * ne), 0);
-        std::vector<int32_t> fill(c.row_ptr.begin(), c.row_ptr.end() - 1);
-        for (int64_t e = e0; e < e1; ++e) {
-            int32_t u = local[h->bcc_uv[2 * e]], v = local[h->bcc_uv[2 * e + 1]];
-            c.col[fill[u]++] = v;
-            c.col[fill[v]++] = u;
-        }
-        c.reach.assign((size_t)c.n, 0);
-        for (auto &pr : reach_lists[b]) c.reach[local[pr.first]] = pr.second;
-        for (int32_t v : c.gid) local[v] = -1;
-        if (use_chains) extract_chains(c);
-        else {
-            c.chain_ptr.assign(1, 0);
-            c.n_free = c.n;
-        }
-        h->comps.push_back(std::move(c));
+    h->comps.resize(blocks.size());
+    int T = host_threads();
+    while (T > 1 && (double)T * h->n * 4.0 > (double)(1ll << 30)) --T;
+    if (h->m < (1 << 16)) T = 1;
+    T = std::max(1, std::min<int>(T, (int)blocks.size()));
+    std::atomic<int64_t> next(0);
+    auto worker = [&]() {
+        std::vector<int32_t> local((size_t)h->n, -1);
+        for (int64_t i; (i = next.fetch_add(1)) < (int64_t)blocks.size();)
+            build_component(h, blocks[i], reach_lists[blocks[i]], use_chains, local, h->comps[i]);
+    };
+    if (T == 1) {
+        worker();
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t) th.emplace_back(worker);
+        for (auto &x : th) x.join();
     }
     h->decomposed = true;
     return BCX_OK;
 }
+
+// Host structure over an edge array (taken over, not copied).
+static int host_from_uv(int32_t n, int64_t m, std::vector<int32_t> &&uv, bcx_host **out) {
+    *out = nullptr;
+    bcx_host *h = new bcx_host();
+    h->n = n;
+    h->m = m;
+    h->uv = std::move(uv);
+    int rc = build_adjacency(h);
+    if (rc) { delete h; return rc; }
+    *out = h;
+    return BCX_OK;
+}
+
+// Read-only view of a whole file: mapped when it is a regular file, else read.
+struct FileView {
+    const char *data = nullptr;
+    size_t size = 0;
+    void *map = nullptr;
+    std::vector<char> copy;
+    bool open(const char *path) {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return false;
+        struct stat st;
+        if (fstat(fd, &st) == 0 && S_ISREG(st.st_mode)) {
+            size = (size_t)st.st_size;
+            if (size == 0) { ::close(fd); data = ""; return true; }
+            map = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (map != MAP_FAILED) {
+                ::close(fd);
+                data = (const char *)map;
+                return true;
+            }
+            map = nullptr;
+        }
+        char tmp[1 << 16];
+        ssize_t k;
+        while ((k = ::read(fd, tmp, sizeof(tmp))) > 0) copy.insert(copy.end(), tmp, tmp + k);
+        ::close(fd);
+        if (k < 0) return false;
+        data = copy.data();
+        size = copy.size();
+        return true;
+    }
+    void close() {
+        if (map) munmap(map, size);
+        map = nullptr;
+        copy.clear();
+        copy.shrink_to_fit();
+    }
+    ~FileView() { close(); }
+};
 
 }  // namespace bcx
 
@@ -361,58 +555,120 @@ extern "C" {
 
 int bcx_host_from_edges(int32_t n, int64_t m, const int32_t *uv, bcx_host **out) {
     if (!out || n < 0 || m < 0 || (m > 0 && !uv)) return BCX_E_ARG;
-    *out = nullptr;
-    bcx_host *h = new bcx_host();
-    h->n = n;
-    h->m = m;
-    h->uv.assign(uv, uv + 2 * m);
-    int rc = bcx::build_adjacency(h);
-    if (rc) { delete h; return rc; }
-    *out = h;
-    return BCX_OK;
+    return bcx::host_from_uv(n, m, std::vector<int32_t>(uv, uv + 2 * m), out);
 }
 
-// Text format of Codes/ReadMe:6-13 / Codes/graph.cpp:44-59.  Parsed with a
-// hand-rolled scanner (fscanf per token is the reference's bottleneck at
-// tens of millions of lines).
+// Text format of Codes/ReadMe:6-13 / Codes/graph.cpp:44-59: `n m`, then m
+// lines `u v w` (w = 1), read token by token as fscanf does; whatever follows
+// the m-th edge is ignored.  The file is mapped and scanned by T threads over
+// ranges cut at whitespace: one pass counts each range's tokens (and where
+// the valid input ends), the second parses them straight into the edge array
+// (fscanf per token is the reference's bottleneck at tens of millions of
+// lines).
 int bcx_host_from_file(const char *path, bcx_host **out) {
     if (!path || !out) return BCX_E_ARG;
     *out = nullptr;
-    FILE *fp = fopen(path, "rb");
-    if (!fp) return BCX_E_IO;
-    std::vector<char> buf;
-    {
-        char tmp[1 << 16];
-        size_t k;
-        while ((k = fread(tmp, 1, sizeof(tmp), fp)) > 0) buf.insert(buf.end(), tmp, tmp + k);
-    }
-    fclose(fp);
-    buf.push_back('\0');
-    const char *p = buf.data();
-    auto next_int = [&](long long &x) -> bool {
-        while (*p && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
-        if (!*p) return false;
+    bcx::FileView f;
+    if (!f.open(path)) return BCX_E_IO;
+    const char *buf = f.data;
+    const size_t len = f.size;
+    auto ws = [](char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; };
+    // one token at p < e (p at a non-space); false if it is not an integer
+    auto parse = [](const char *&p, const char *e, long long &x) -> bool {
         bool neg = false;
-        if (*p == '-' || *p == '+') { neg = (*p == '-'); ++p; }
+        if (*p == '-' || *p == '+') {
+            neg = (*p == '-');
+            if (++p >= e) return false;
+        }
         if (*p < '0' || *p > '9') return false;
         long long v = 0;
-        while (*p >= '0' && *p <= '9') { v = v * 10 + (*p - '0'); ++p; }
+        int digits = 0;
+        while (p < e && *p >= '0' && *p <= '9') {
+            if (++digits > 18) return false;
+            v = v * 10 + (*p - '0');
+            ++p;
+        }
         x = neg ? -v : v;
         return true;
     };
-    long long n, m;
-    if (!next_int(n) || !next_int(m) || n < 0 || m < 0 || n > INT32_MAX) return BCX_E_IO;
-    std::vector<int32_t> uv;
-    uv.reserve((size_t)(2 * m));
-    for (long long e = 0; e < m; ++e) {
-        long long u, v, w;
-        if (!next_int(u) || !next_int(v) || !next_int(w)) return BCX_E_IO;
-        if (u < 0 || u >= n || v < 0 || v >= n) return BCX_E_IO;
-        if (w != 1) return BCX_E_IO;  // unit weights only (Codes/ReadMe:6-9)
-        uv.push_back((int32_t)u);
-        uv.push_back((int32_t)v);
+    int T = bcx::host_threads();
+    if (len < (1u << 20)) T = 1;
+    // range t is [cut[t], cut[t+1]); inner cuts sit just after whitespace
+    std::vector<size_t> cut((size_t)T + 1);
+    for (int t = 0; t <= T; ++t) {
+        size_t c = len * (size_t)t / (size_t)T;
+        while (c < len && c > 0 && !ws(buf[c - 1])) ++c;
+        cut[t] = c;
     }
-    return bcx_host_from_edges((int32_t)n, (int64_t)m, uv.data(), out);
+    // scan range t: every valid token goes to emit(index within the range,
+    // value); returns the count and whether the valid input stops inside
+    auto scan = [&](int t, auto emit) -> std::pair<size_t, bool> {
+        const char *p = buf + cut[t], *e = buf + cut[t + 1];
+        size_t k = 0;
+        for (;;) {
+            while (p < e && ws(*p)) ++p;
+            if (p >= e) return {k, false};
+            long long x;
+            if (!parse(p, e, x)) return {k, true};
+            emit(k++, x);
+            // a token glued to something else ends the valid input after it
+            if (p < e && !ws(*p)) return {k, true};
+        }
+    };
+    std::vector<size_t> cnt((size_t)T, 0);
+    std::vector<uint8_t> stopped((size_t)T, 0);
+    std::vector<long long> head;  // the first tokens (n, m)
+    bcx::parallel_tasks(T, [&](int t) {
+        auto r = scan(t, [&](size_t k, long long x) {
+            if (t == 0 && k < 2) head.push_back(x);
+        });
+        cnt[t] = r.first;
+        stopped[t] = r.second;
+    });
+    // the valid tokens are those up to the first range that stopped
+    std::vector<size_t> start((size_t)T + 1, 0);
+    size_t valid = 0;
+    for (int t = 0; t < T; ++t) {
+        start[t] = valid;
+        valid += cnt[t];
+        if (stopped[t]) {
+            for (int u = t + 1; u <= T; ++u) start[u] = valid;
+            break;
+        }
+        start[t + 1] = valid;
+    }
+    if (head.size() < 2) {  // n and m may sit in a later range of a tiny first one
+        head.clear();
+        for (int t = 0; t < T && head.size() < 2; ++t)
+            scan(t, [&](size_t, long long x) { if (head.size() < 2) head.push_back(x); });
+        if (head.size() < 2 || valid < 2) return BCX_E_IO;
+    }
+    const long long n = head[0], m = head[1];
+    if (n < 0 || m < 0 || n > INT32_MAX) return BCX_E_IO;
+    if ((long double)valid < 2.0L + 3.0L * (long double)m) return BCX_E_IO;
+    std::vector<int32_t> uv((size_t)(2 * m));
+    std::atomic<int> bad(0);
+    const size_t last = 2 + 3 * (size_t)m;  // one past the last token used
+    bcx::parallel_tasks(T, [&](int t) {
+        if (start[t] >= last || (t > 0 && start[t] == start[t - 1] && stopped[t - 1])) return;
+        int32_t *dst = uv.data();
+        bool ok = true;
+        scan(t, [&](size_t k, long long x) {
+            const size_t i = start[t] + k;  // global token index
+            if (i < 2 || i >= last) return;
+            const size_t e = (i - 2) / 3, f = (i - 2) % 3;
+            if (f < 2) {
+                if (x < 0 || x >= n) ok = false;
+                else dst[2 * e + f] = (int32_t)x;
+            } else if (x != 1) {
+                ok = false;  // unit weights only (Codes/ReadMe:6-9)
+            }
+        });
+        if (!ok) bad = 1;
+    });
+    if (bad.load()) return BCX_E_IO;
+    f.close();
+    return bcx::host_from_uv((int32_t)n, (int64_t)m, std::move(uv), out);
 }
 
 void bcx_host_free(bcx_host *h) { delete h; }

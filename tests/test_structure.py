@@ -123,6 +123,82 @@ def test_file_reader_matches_edges(tmp_path):
         bcx.Host.from_file(str(p))
 
 
+def test_input_check_messages():
+    # first vertex whose list repeats a neighbour, reported at the repeat
+    with pytest.raises(bcx.BcxError, match="duplicate edge 1 2"):
+        bcx.Host.from_edges(4, [(0, 1), (1, 2), (2, 0), (2, 1), (2, 3)]).decompose()
+    with pytest.raises(bcx.BcxError, match="self loop on vertex 2"):
+        bcx.Host.from_edges(4, [(0, 1), (2, 2), (1, 2), (3, 3)]).decompose()
+    with pytest.raises(bcx.BcxError, match=r"disconnected \(2 of 5 vertices reachable from 0\)"):
+        bcx.Host.from_edges(5, [(0, 1), (2, 3), (3, 4), (4, 2)]).decompose()
+
+
+def test_file_reader_token_rules(tmp_path):
+    """Token-by-token reading (fscanf semantics of Codes/graph.cpp:44-59):
+    line breaks are whitespace, anything after the m-th edge is ignored, a
+    short or malformed edge list is an error."""
+    p = tmp_path / "g.txt"
+
+    def read(text):
+        p.write_text(text)
+        return bcx.Host.from_file(str(p))
+
+    ok = {
+        "plain": "3 3\n0 1 1\n1 2 1\n2 0 1\n",
+        "no final newline": "3 3\n0 1 1\n1 2 1\n2 0 1",
+        "split lines": "3 3 0\n1 1 1 2\n1\n2 0 1\n",
+        "signs and tabs": "+3\t3\r\n0 +1 1\n1 2 +1\n2 0 1\n",
+        "trailing junk": "3 3\n0 1 1\n1 2 1\n2 0 1\nthis is not an edge\n",
+        "glued last token": "3 3\n0 1 1\n1 2 1\n2 0 1x\n",
+    }
+    for name, text in ok.items():
+        h = read(text).decompose()
+        assert (h.n, h.m, h.num_bcc()) == (3, 3, 1), name
+        assert sorted(map(tuple, np.sort(h.bcc_edges(0), axis=1).tolist())) == [(0, 1), (0, 2), (1, 2)], name
+    bad = {
+        "short": "3 3\n0 1 1\n1 2 1\n",
+        "letter inside": "3 3\n0 1 1\n1 x 1\n2 0 1\n",
+        "id out of range": "3 3\n0 1 1\n1 3 1\n2 0 1\n",
+        "negative id": "3 3\n0 1 1\n1 -2 1\n2 0 1\n",
+        "weight 2": "3 3\n0 1 1\n1 2 2\n2 0 1\n",
+        "empty": "",
+        "header only n": "3",
+    }
+    for name, text in bad.items():
+        with pytest.raises(bcx.BcxError):
+            read(text)
+            pytest.fail(name)
+
+
+def _structure_arrays(h):
+    out = [h.cut_term(), h.articulation()]
+    for c in h.components():
+        out += [np.asarray(x) for x in (c.gid, c.row_ptr, c.col, c.reach, c.chain_ptr, c.chain_nodes, c.chain_ends)]
+    return out
+
+
+def test_threaded_reader_and_structure_match_single_thread(tmp_path, monkeypatch):
+    """The text reader scans whitespace-cut ranges on several threads and the
+    adjacency, input check and component build run threaded: every array must
+    equal the single-thread result and the in-memory edge list's."""
+    for cfg, scale in ((2, 0.03), (3, 0.002)):
+        n, uv = bcx.generate(cfg, 0, scale)
+        p = tmp_path / f"g{cfg}.txt"
+        with open(p, "w") as f:
+            f.write(f"{n} {len(uv)}\n")
+            np.savetxt(f, np.column_stack([uv, np.ones(len(uv), np.int64)]), fmt="%d")
+        assert p.stat().st_size > (1 << 20)  # large enough for the threaded scan
+        results = []
+        for threads in ("1", "3", "7"):
+            monkeypatch.setenv("BCX_HOST_THREADS", threads)
+            results.append(_structure_arrays(bcx.Host.from_file(str(p)).decompose()))
+            results.append(_structure_arrays(bcx.Host.from_edges(n, uv).decompose()))
+        for r in results[1:]:
+            assert len(r) == len(results[0])
+            for a, b in zip(r, results[0]):
+                np.testing.assert_array_equal(a, b)
+
+
 def test_generators_deterministic_and_simple():
     for cfg in (1, 2, 3, 4, 5):
         n1, uv1 = bcx.generate(cfg, 0, 0.002 if cfg != 1 else 0.2)
