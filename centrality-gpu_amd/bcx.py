@@ -96,6 +96,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_double), _i32p]),
     "bcx_generate": (C.c_int, [C.c_int32, C.c_uint64, C.c_double, _i32p, _i64p,
                                C.POINTER(_i32p)]),
+    "bcx_host_save": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "bcx_host_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
     "bcx_free": (None, [C.c_void_p]),
 }
 
@@ -208,6 +210,24 @@ class Host:
             raise BcxError(rc, "bad edge list")
         return Host(h.value)
 
+    def save(self, path, source=None):
+        """Writes the reduced structure to a cache file (see bcx_host_save)."""
+        rc = lib().bcx_host_save(self.h, str(path).encode(), None if source is None else str(source).encode())
+        if rc:
+            raise BcxError(rc, f"cannot write structure cache {path}")
+        return self
+
+    @staticmethod
+    def load(path, source=None, use_chains=True):
+        """A reduced host from a cache file; BcxError when it is damaged,
+        stale for `source` or written with another chains setting."""
+        h = C.c_void_p()
+        rc = lib().bcx_host_load(str(path).encode(), None if source is None else str(source).encode(),
+                                 int(use_chains), C.byref(h))
+        if rc:
+            raise BcxError(rc, f"structure cache {path} is not usable")
+        return Host(h.value)
+
     def __del__(self):
         if getattr(self, "h", None) and self.h.value and _lib is not None:
             _lib.bcx_host_free(self.h)
@@ -230,12 +250,28 @@ class Host:
     def num_bcc(self):
         return lib().bcx_host_num_bcc(self.h)
 
+    def bcc_size(self, i):
+        """(vertices, undirected edges) of BCC i."""
+        n = C.c_int32()
+        e = C.c_int64()
+        rc = lib().bcx_host_bcc_size(self.h, i, C.byref(n), C.byref(e))
+        if rc:
+            raise BcxError(rc, f"no BCC {i}")
+        return n.value, e.value
+
+    def num_components(self):
+        return lib().bcx_host_num_components(self.h)
+
     def bcc_edges(self, i):
         n = C.c_int32()
         e = C.c_int64()
-        lib().bcx_host_bcc_size(self.h, i, C.byref(n), C.byref(e))
+        rc = lib().bcx_host_bcc_size(self.h, i, C.byref(n), C.byref(e))
+        if rc:
+            raise BcxError(rc, f"no BCC {i}")
         uv = np.zeros(2 * e.value, np.int32)
-        lib().bcx_host_bcc_edges(self.h, i, _ptr(uv))
+        rc = lib().bcx_host_bcc_edges(self.h, i, _ptr(uv))
+        if rc:
+            raise BcxError(rc, "BCC edge lists are not available (host loaded from a structure cache?)")
         return uv.reshape(-1, 2)
 
     def articulation(self):

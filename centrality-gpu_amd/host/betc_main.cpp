@@ -19,6 +19,10 @@
 //                        seeded; all sources when the component is smaller),
 //                        each component's dependency sum scaled by n_c / K
 //   --seed S             seed of --sample (default 1)
+//   --cache              keep the reduced structure in <edge_list_file>.bcx
+//                        and reuse it while the text file's size and
+//                        modification time are unchanged (skips the text
+//                        parse and the reduction on repeated runs)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -54,7 +58,7 @@ static std::vector<int32_t> sample_sources(int32_t n, int64_t k, uint64_t seed, 
 
 int main(int argc, char **argv) {
     const char *in = nullptr, *bc_out = nullptr, *bc_exact = nullptr;
-    bool batched = false, fp32 = false, chains = true;
+    bool batched = false, fp32 = false, chains = true, cache = false;
     int64_t sample = 0;
     uint64_t seed = 1;
     for (int i = 1; i < argc; ++i) {
@@ -63,6 +67,7 @@ int main(int argc, char **argv) {
         else if (!std::strcmp(argv[i], "--batched")) batched = true;
         else if (!std::strcmp(argv[i], "--fp32")) fp32 = true;
         else if (!std::strcmp(argv[i], "--no-chains")) chains = false;
+        else if (!std::strcmp(argv[i], "--cache")) cache = true;
         else if (!std::strcmp(argv[i], "--sample") && i + 1 < argc) sample = std::atoll(argv[++i]);
         else if (!std::strcmp(argv[i], "--seed") && i + 1 < argc) seed = std::strtoull(argv[++i], nullptr, 10);
         else if (!in) in = argv[i];
@@ -72,14 +77,26 @@ int main(int argc, char **argv) {
         return 1;
     }
     bcx_host *h = nullptr;
-    int rc = bcx_host_from_file(in, &h);
-    if (rc == BCX_E_IO && !h) {
-        std::printf("Input file cannot be opened\n");
-        return 2;
+    int rc = BCX_E_IO;
+    const std::string cache_path = std::string(in) + ".bcx";
+    if (cache && bcx_host_load(cache_path.c_str(), in, chains ? 1 : 0, &h) == BCX_OK) {
+        std::fprintf(stderr, "betc: structure read from %s\n", cache_path.c_str());
+    } else {
+        rc = bcx_host_from_file(in, &h);
+        if (rc == BCX_E_IO && !h) {
+            std::printf("Input file cannot be opened\n");
+            return 2;
+        }
+        if (rc) return die("read", "malformed input", 2);
+        rc = bcx_host_decompose(h, chains ? 1 : 0, 1);
+        if (rc) return die("decompose", bcx_host_last_error(h), 3);
+        if (cache) {
+            if (bcx_host_save(h, cache_path.c_str(), in) == BCX_OK)
+                std::fprintf(stderr, "betc: structure written to %s\n", cache_path.c_str());
+            else
+                std::fprintf(stderr, "betc: cannot write %s (continuing without it)\n", cache_path.c_str());
+        }
     }
-    if (rc) return die("read", "malformed input", 2);
-    rc = bcx_host_decompose(h, chains ? 1 : 0, 1);
-    if (rc) return die("decompose", bcx_host_last_error(h), 3);
 
     const int32_t n = bcx_host_num_vertices(h);
     const int32_t nb = bcx_host_num_bcc(h);

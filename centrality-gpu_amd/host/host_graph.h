@@ -34,6 +34,8 @@ struct bcx_host {
     std::vector<int32_t> adj;         // [2m] neighbours in file order
 
     bool decomposed = false;
+    bool chains = true;               // use_chains of the decomposition
+    bool cached = false;              // loaded from a structure cache (no edge lists)
     std::vector<int64_t> bcc_ptr;     // [n_bcc+1] offsets (in edges) into bcc_uv
     std::vector<int32_t> bcc_uv;      // edges of all BCCs, pop order, global ids
     std::vector<int32_t> bcc_nv;      // vertices per BCC

@@ -446,6 +446,11 @@ static void build_component(const bcx_host *h, int32_t b, const std::vector<std:
 }
 
 int decompose(bcx_host *h, bool use_chains, bool check_input) {
+    if (h->cached) {  // already reduced; the edge list was not stored
+        if (use_chains == h->chains) return BCX_OK;
+        h->err = "host loaded from a structure cache written with another chains setting";
+        return BCX_E_ARG;
+    }
     if (h->adj_ptr.size() != (size_t)h->n + 1) {
         int rc = build_adjacency(h);
         if (rc) return rc;
@@ -489,6 +494,7 @@ int decompose(bcx_host *h, bool use_chains, bool check_input) {
         for (auto &x : th) x.join();
     }
     h->decomposed = true;
+    h->chains = use_chains;
     return BCX_OK;
 }
 
@@ -696,6 +702,7 @@ int bcx_host_bcc_size(const bcx_host *h, int32_t i, int32_t *n, int64_t *edges) 
 
 int bcx_host_bcc_edges(const bcx_host *h, int32_t i, int32_t *uv) {
     if (!h || !h->decomposed || !uv || i < 0 || i >= (int32_t)h->bcc_nv.size()) return BCX_E_ARG;
+    if (h->cached) return BCX_E_ARG;  // edge lists are not in the cache
     std::memcpy(uv, h->bcc_uv.data() + 2 * h->bcc_ptr[i],
                 sizeof(int32_t) * 2 * (h->bcc_ptr[i + 1] - h->bcc_ptr[i]));
     return BCX_OK;
@@ -734,6 +741,177 @@ int bcx_host_component_counts(const bcx_host *h, int32_t c, int32_t *n_free,
     if (n_free) *n_free = k.n_free;
     if (n_endpoints) *n_endpoints = k.n_endpoints;
     if (n_chain_vertices) *n_chain_vertices = k.n_chain_vertices;
+    return BCX_OK;
+}
+
+}  // extern "C"
+
+// ---- structure cache ---------------------------------------------------
+// Little-endian file: the header below, then (each array 8-byte aligned)
+// bcc_ptr[nb+1] i64, bcc_nv[nb] i32, is_art[n] u8, cut[n] i64, and per
+// component a record of counts followed by gid, row_ptr, col, reach,
+// chain_ptr, chain_nodes, chain_ends; an end marker closes the file.
+namespace {
+constexpr char kMagic[8] = {'B', 'C', 'X', 'S', 'T', 'R', 'U', 'C'};
+constexpr uint32_t kVersion = 1;
+constexpr uint64_t kEnd = 0x444E4558434221ull;  // "!BCXEND"
+struct CacheHeader {
+    char magic[8];
+    uint32_t version, chains;
+    int64_t src_size, src_mtime_ns;
+    int64_t m;
+    int32_t n, nb, ncomp, pad;
+};
+struct CompRecord {
+    int32_t bcc, n, n_free, n_endpoints, n_chain_vertices, n_chains;
+    int64_t arcs, n_chain_nodes;
+};
+
+bool source_id(const char *path, int64_t &size, int64_t &mtime_ns) {
+    size = mtime_ns = 0;
+    if (!path) return true;
+    struct stat st;
+    if (stat(path, &st) != 0) return false;
+    size = (int64_t)st.st_size;
+    mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000ll + st.st_mtim.tv_nsec;
+    return true;
+}
+
+struct Writer {
+    FILE *f;
+    int64_t at = 0;
+    bool ok = true;
+    void raw(const void *p, size_t nbytes) {
+        if (nbytes && fwrite(p, 1, nbytes, f) != nbytes) ok = false;
+        at += (int64_t)nbytes;
+    }
+    template <class T> void arr(const T *p, size_t count) {
+        raw(p, sizeof(T) * count);
+        static const char zero[8] = {0};
+        raw(zero, (size_t)((8 - (at & 7)) & 7));
+    }
+};
+
+struct Reader {
+    const char *p, *e;
+    bool ok = true;
+    template <class T> bool arr(std::vector<T> &out, int64_t count) {
+        const int64_t nbytes = (int64_t)sizeof(T) * count;
+        const int64_t padded = (nbytes + 7) & ~7ll;
+        if (!ok || count < 0 || padded > e - p) return ok = false;
+        out.resize((size_t)count);
+        if (nbytes) std::memcpy(out.data(), p, (size_t)nbytes);
+        p += padded;
+        return true;
+    }
+    template <class T> bool one(T &x) {
+        if (!ok || (int64_t)sizeof(T) > e - p) return ok = false;
+        std::memcpy(&x, p, sizeof(T));
+        p += (sizeof(T) + 7) & ~(size_t)7;
+        return true;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int bcx_host_save(const bcx_host *h, const char *path, const char *source_path) {
+    if (!h || !path || !h->decomposed || h->cached) return BCX_E_ARG;
+    CacheHeader hd{};
+    std::memcpy(hd.magic, kMagic, 8);
+    hd.version = kVersion;
+    hd.chains = h->chains ? 1 : 0;
+    if (!source_id(source_path, hd.src_size, hd.src_mtime_ns)) return BCX_E_IO;
+    hd.m = h->m;
+    hd.n = h->n;
+    hd.nb = (int32_t)h->bcc_nv.size();
+    hd.ncomp = (int32_t)h->comps.size();
+    // written beside the target and renamed, so a reader never sees a torn file
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return BCX_E_IO;
+    std::vector<char> vbuf(1 << 22);
+    setvbuf(f, vbuf.data(), _IOFBF, vbuf.size());
+    Writer w{f};
+    w.arr(&hd, 1);
+    w.arr(h->bcc_ptr.data(), h->bcc_ptr.size());
+    w.arr(h->bcc_nv.data(), h->bcc_nv.size());
+    w.arr(h->is_art.data(), h->is_art.size());
+    w.arr(h->cut.data(), h->cut.size());
+    for (const Component &c : h->comps) {
+        CompRecord r{c.bcc, c.n, c.n_free, c.n_endpoints, c.n_chain_vertices,
+                     (int32_t)(c.chain_ends.size() / 2), (int64_t)c.col.size(), (int64_t)c.chain_nodes.size()};
+        w.arr(&r, 1);
+        w.arr(c.gid.data(), c.gid.size());
+        w.arr(c.row_ptr.data(), c.row_ptr.size());
+        w.arr(c.col.data(), c.col.size());
+        w.arr(c.reach.data(), c.reach.size());
+        w.arr(c.chain_ptr.data(), c.chain_ptr.size());
+        w.arr(c.chain_nodes.data(), c.chain_nodes.size());
+        w.arr(c.chain_ends.data(), c.chain_ends.size());
+    }
+    w.arr(&kEnd, 1);
+    const bool ok = w.ok && fclose(f) == 0;
+    if (!ok || rename(tmp.c_str(), path) != 0) {
+        remove(tmp.c_str());
+        return BCX_E_IO;
+    }
+    return BCX_OK;
+}
+
+int bcx_host_load(const char *path, const char *source_path, int use_chains, bcx_host **out) {
+    if (!path || !out) return BCX_E_ARG;
+    *out = nullptr;
+    bcx::FileView f;
+    if (!f.open(path)) return BCX_E_IO;
+    Reader r{f.data, f.data + f.size};
+    CacheHeader hd;
+    if (!r.one(hd) || std::memcmp(hd.magic, kMagic, 8) || hd.version != kVersion) return BCX_E_IO;
+    int64_t size, mtime;
+    if (!source_id(source_path, size, mtime)) return BCX_E_IO;
+    if (source_path && (size != hd.src_size || mtime != hd.src_mtime_ns)) return BCX_E_IO;
+    if ((int)hd.chains != (use_chains ? 1 : 0) || hd.n < 0 || hd.nb < 0 || hd.ncomp < 0 || hd.m < 0)
+        return BCX_E_IO;
+    bcx_host *h = new bcx_host();
+    h->n = hd.n;
+    h->m = hd.m;
+    h->chains = hd.chains != 0;
+    h->cached = true;
+    auto fail = [&]() { delete h; return BCX_E_IO; };
+    if (!r.arr(h->bcc_ptr, (int64_t)hd.nb + 1) || !r.arr(h->bcc_nv, hd.nb) || !r.arr(h->is_art, hd.n) ||
+        !r.arr(h->cut, hd.n))
+        return fail();
+    h->comps.resize((size_t)hd.ncomp);
+    for (Component &c : h->comps) {
+        CompRecord cr;
+        if (!r.one(cr) || cr.n < 0 || cr.n_chains < 0 || cr.arcs < 0 || cr.n_chain_nodes < 0) return fail();
+        c.bcc = cr.bcc;
+        c.n = cr.n;
+        c.n_free = cr.n_free;
+        c.n_endpoints = cr.n_endpoints;
+        c.n_chain_vertices = cr.n_chain_vertices;
+        if (!r.arr(c.gid, cr.n) || !r.arr(c.row_ptr, (int64_t)cr.n + 1) || !r.arr(c.col, cr.arcs) ||
+            !r.arr(c.reach, cr.n) || !r.arr(c.chain_ptr, (int64_t)cr.n_chains + 1) ||
+            !r.arr(c.chain_nodes, cr.n_chain_nodes) || !r.arr(c.chain_ends, 2 * (int64_t)cr.n_chains))
+            return fail();
+        // the views handed to the GPU index with these: keep them in range
+        if (c.row_ptr[0] != 0 || c.row_ptr[cr.n] != cr.arcs) return fail();
+        for (int32_t v = 0; v < cr.n; ++v)
+            if (c.row_ptr[v + 1] < c.row_ptr[v] || c.gid[v] < 0 || c.gid[v] >= hd.n) return fail();
+        for (int32_t x : c.col)
+            if (x < 0 || x >= cr.n) return fail();
+        if (c.chain_ptr[0] != 0 || c.chain_ptr[cr.n_chains] != cr.n_chain_nodes) return fail();
+        for (int32_t k = 0; k < cr.n_chains; ++k)
+            if (c.chain_ptr[k + 1] < c.chain_ptr[k]) return fail();
+        for (int32_t x : c.chain_nodes)
+            if (x < 0 || x >= cr.n) return fail();
+        for (int32_t x : c.chain_ends)
+            if (x < 0 || x >= cr.n) return fail();
+    }
+    uint64_t end = 0;
+    if (!r.one(end) || end != kEnd || r.p != r.e) return fail();
+    h->decomposed = true;
+    *out = h;
     return BCX_OK;
 }
 

@@ -114,6 +114,21 @@ int  bcx_host_component_counts(const bcx_host *h, int32_t c,
                                int32_t *n_free, int32_t *n_endpoints,
                                int32_t *n_chain_vertices);
 
+/* Structure cache (no counterpart in the reference, whose text parse and
+ * reduction rerun on every call; this is for repeated runs on one input):
+ * the reduced host in a binary file.  bcx_host_save needs a decomposed host
+ * and records source_path's size and modification time (none when NULL);
+ * the file is written beside `path` and renamed into place.
+ * bcx_host_load returns BCX_E_IO when the file is damaged or of another
+ * format version, when source_path is given and its size or modification
+ * time differ from the recorded ones, or when use_chains differs from the
+ * saved decomposition: the caller then reads and reduces the text again.
+ * A loaded host answers every query except bcx_host_bcc_edges (BCX_E_ARG:
+ * edge lists are not stored); bcx_host_decompose on it is a no-op for the
+ * same use_chains.                                                         */
+int  bcx_host_save(const bcx_host *h, const char *path, const char *source_path);
+int  bcx_host_load(const char *path, const char *source_path, int use_chains, bcx_host **out);
+
 /* ======================================================================= */
 /* 2. GPU engine                                                            */
 /* ======================================================================= */

@@ -263,3 +263,35 @@ def test_cli_sampled(tmp_path):
     assert abs(got.sum() - want.sum()) / want.sum() < 0.05
     top = np.argsort(-want)[:20]
     assert np.all(np.abs(got[top] - want[top]) / want[top] < 0.5)
+
+
+def test_cli_structure_cache(tmp_path):
+    """--cache: the first run writes <input>.bcx, the second reads it; both
+    print the same structure lines and the same BC (equal to the oracle's);
+    a changed input file is read again."""
+    n, uv = bcx.generate(3, 0, 0.001)
+    uv = [tuple(map(int, e)) for e in np.asarray(uv).reshape(-1, 2)]
+    src = tmp_path / "g.txt"
+    _write_edges(src, n, uv)
+    exe = os.path.join(bcx.HERE, "betc")
+    rp, col = oracle.csr_from_edges(n, uv)
+    want = oracle.brandes_bc(rp, col, threads=16)
+    runs = []
+    for k in range(3):
+        if k == 2:
+            st = src.stat()
+            os.utime(src, ns=(st.st_atime_ns, st.st_mtime_ns + 1_000_000_000))
+        out = tmp_path / f"bc{k}.txt"
+        r = subprocess.run([exe, str(src), "--batched", "--cache", "--bc-out-exact", str(out)],
+                           capture_output=True, text=True, cwd=tmp_path, timeout=300)
+        assert r.returncode == 0, r.stderr
+        print("run", k, r.stderr.strip(), flush=True)
+        mx, _, zeros = relerr(np.loadtxt(out)[:, 1], want)
+        assert mx < 1e-9 and zeros
+        runs.append((r.stderr, _structure(r.stdout), np.loadtxt(out)[:, 1]))
+    assert "structure written to" in runs[0][0]
+    assert "structure read from" in runs[1][0]
+    assert "structure written to" in runs[2][0]  # stale cache rebuilt
+    assert runs[0][1] == runs[1][1] == runs[2][1]
+    np.testing.assert_allclose(runs[1][2], runs[0][2], rtol=1e-12, atol=0)  # FP64 sums, tile order may differ
+    assert (tmp_path / "g.txt.bcx").exists()

@@ -199,6 +199,71 @@ def test_threaded_reader_and_structure_match_single_thread(tmp_path, monkeypatch
                 np.testing.assert_array_equal(a, b)
 
 
+def _write_text(path, n, uv):
+    with open(path, "w") as f:
+        f.write(f"{n} {len(uv)}\n")
+        np.savetxt(f, np.column_stack([uv, np.ones(len(uv), np.int64)]), fmt="%d")
+
+
+def test_structure_cache_round_trip(tmp_path):
+    """A saved and reloaded host answers every query as the original did."""
+    for cfg, scale in ((1, 1.0), (3, 0.002), (4, 0.05)):
+        n, uv = bcx.generate(cfg, 0, scale)
+        src = tmp_path / f"g{cfg}.txt"
+        _write_text(src, n, uv)
+        h = bcx.Host.from_file(str(src)).decompose()
+        cache = tmp_path / f"g{cfg}.txt.bcx"
+        h.save(cache, source=src)
+        g = bcx.Host.load(cache, source=src)
+        assert (g.n, g.m, g.num_bcc(), g.num_components()) == (h.n, h.m, h.num_bcc(), h.num_components())
+        for a, b in zip(_structure_arrays(g), _structure_arrays(h)):
+            np.testing.assert_array_equal(a, b)
+        for i in range(h.num_bcc()):
+            assert g.bcc_size(i) == h.bcc_size(i)
+        for c in range(h.num_components()):
+            assert g.component_counts(c) == h.component_counts(c)
+        g.decompose()  # same chains setting: nothing to do
+        with pytest.raises(bcx.BcxError):
+            g.bcc_edges(0)  # edge lists are not stored
+        if cfg == 1:
+            np.testing.assert_array_equal(oracle.pipeline_bc(g, threads=1), oracle.pipeline_bc(h, threads=1))
+
+
+def test_structure_cache_rejects_stale_or_damaged(tmp_path):
+    n, uv = bcx.generate(1, 0, 1.0)
+    src = tmp_path / "g.txt"
+    _write_text(src, n, uv)
+    cache = tmp_path / "g.txt.bcx"
+    bcx.Host.from_file(str(src)).decompose().save(cache, source=src)
+    bcx.Host.load(cache, source=src)
+    bcx.Host.load(cache)  # no source: format checks only
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.load(cache, source=src, use_chains=False)
+    st = src.stat()
+    bcx.os.utime(src, ns=(st.st_atime_ns, st.st_mtime_ns + 1_000_000_000))
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.load(cache, source=src)  # the text changed after the cache was written
+    raw = cache.read_bytes()
+    for bad in (raw[:-8], raw[: len(raw) // 2], raw + b"\0" * 8, b"BCXSTRUX" + raw[8:], b""):
+        damaged = tmp_path / "damaged.bcx"
+        damaged.write_bytes(bad)
+        with pytest.raises(bcx.BcxError):
+            bcx.Host.load(damaged)
+    # a column index out of range is caught before any view is handed out
+    h = bcx.Host.load(cache)
+    col0 = np.asarray(h.components()[0].col)
+    arr = np.frombuffer(raw, np.uint8).copy()
+    needle = col0[:16].astype("<i4").tobytes()
+    at = raw.find(needle)
+    assert at > 0
+    arr[at:at + 4] = np.frombuffer(np.int32(10**8).astype("<i4").tobytes(), np.uint8)
+    damaged.write_bytes(arr.tobytes())
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.load(damaged)
+    with pytest.raises(bcx.BcxError):
+        bcx.Host.from_edges(3, [(0, 1), (1, 2), (2, 0)]).save(tmp_path / "x.bcx")  # not decomposed
+
+
 def test_generators_deterministic_and_simple():
     for cfg in (1, 2, 3, 4, 5):
         n1, uv1 = bcx.generate(cfg, 0, 0.002 if cfg != 1 else 0.2)
