@@ -140,7 +140,9 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
     if (ldi(&b.ctr[3]) >= 0) return;
     const int q0 = b.qoff[l];
     const int q1 = min(ldi(&b.ctr[0]), b.qcap);
-    const bool stop = (q1 == q0) || ldi(&b.ctr[2]) != 0;
+    // an empty level ends the forward; so do full lists and a count that left
+    // the sigma word (the host reruns the tile either way)
+    const bool stop = (q1 == q0) || ldi(&b.ctr[2]) != 0 || ldi(&b.ctr[1]) != 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         b.qoff[l + 1] = q1;
         if (stop) b.ctr[3] = l - 1;

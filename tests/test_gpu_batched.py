@@ -97,6 +97,34 @@ def test_fp64_sigma_grid(engine):
     assert mx < 1e-9 and zeros
 
 
+@pytest.mark.parametrize("prec", [bcx.PREC_FP64, bcx.PREC_FP32C])
+def test_sigma_word_escalation_through_hub_slices(engine, prec):
+    """A hub of degree k = 260 (its sigma summed by arc slices with atomics)
+    reaches sigma = k^4 = 4.6e9 > 2^32 from vertex 0 through four complete
+    bipartite layers: the tile leaves 32-bit words at the hub's level, its
+    forward stops there and it is rerun with 64-bit words.
+    Layers: 0 - L1 = L2 = L3 = L4 - hub (k vertices per layer)."""
+    k = 260
+    layers = [list(range(1 + i * k, 1 + (i + 1) * k)) for i in range(4)]
+    hub = 1 + 4 * k
+    edges = [(0, a) for a in layers[0]]
+    for i in range(3):
+        edges += [(a, b) for a in layers[i] for b in layers[i + 1]]
+    edges += [(d, hub) for d in layers[3]]
+    c = comps_of(hub + 1, edges)[0]
+    assert c.n == hub + 1
+    gid = list(c.gid)
+    _, su, _, _, _ = oracle.source(c.row_ptr, c.col, c.reach, gid.index(0))
+    assert int(su[gid.index(hub)]) == k ** 4 > 2 ** 32
+    for s in (0, hub, 5, 3 * k):
+        check_source(engine, c, gid.index(s), prec)
+    want = oracle.component_bc(c.row_ptr, c.col, c.reach)
+    flags = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_NO_WG_KERNEL
+    got, _ = engine.bc([c], path=B, precision=prec, flags=flags)
+    mx, _, zeros = relerr(got[0], want)
+    assert mx < (1e-6 if prec == bcx.PREC_FP32C else 1e-9) and zeros
+
+
 def test_threshold_hook_and_subset(engine):
     n, uv = bcx.generate(4, 0, 0.05)
     c = max(comps_of(n, uv), key=lambda c: c.n)
