@@ -472,8 +472,15 @@ int decompose(bcx_host *h, bool use_chains, bool check_input) {
     // are built independently; each thread keeps its own global -> local map
     const int32_t nb = (int32_t)h->bcc_nv.size();
     std::vector<int32_t> blocks;
-    for (int32_t b = 0; b < nb; ++b)
-        if (h->bcc_nv[b] > 2) blocks.push_back(b);
+    for (int32_t b = 0; b < nb; ++b) {
+        if (h->bcc_nv[b] <= 2) continue;
+        // component CSR offsets are int32 (bcx_component.row_ptr)
+        if (2 * (h->bcc_ptr[b + 1] - h->bcc_ptr[b]) > (int64_t)INT32_MAX) {
+            h->err = "biconnected component " + std::to_string(b) + " has more than 2^31-1 arcs";
+            return BCX_E_GRAPH;
+        }
+        blocks.push_back(b);
+    }
     h->comps.clear();
     h->comps.resize(blocks.size());
     int T = host_threads();
