@@ -24,6 +24,7 @@
 //                        modification time are unchanged (skips the text
 //                        parse and the reduction on repeated runs)
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -54,6 +55,44 @@ static std::vector<int32_t> sample_sources(int32_t n, int64_t k, uint64_t seed, 
     s.assign(all.begin(), all.begin() + k);
     std::sort(s.begin(), s.end());
     return s;
+}
+
+// "%d %f\n" (or "%d %.17g\n") per vertex.  std::to_chars with a precision
+// produces exactly what printf does for the same conversion; chunks of
+// vertices are formatted on several threads and written in order.
+static bool write_bc(FILE *f, const std::vector<double> &bc, bool exact) {
+    const int64_t n = (int64_t)bc.size();
+    const int64_t chunk = 1 << 16;
+    const int64_t nch = (n + chunk - 1) / chunk;
+    unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::string> out((size_t)std::min<int64_t>(nch, 4 * (int64_t)T));
+    for (int64_t c0 = 0; c0 < nch; c0 += (int64_t)out.size()) {
+        const int64_t c1 = std::min<int64_t>(nch, c0 + (int64_t)out.size());
+        auto fmt = [&](int64_t c) {
+            std::string &s = out[(size_t)(c - c0)];
+            s.clear();
+            char buf[400];  // any double in %f fits
+            for (int64_t v = c * chunk; v < std::min(n, (c + 1) * chunk); ++v) {
+                char *p = std::to_chars(buf, buf + sizeof(buf), (int32_t)v).ptr;
+                *p++ = ' ';
+                p = exact ? std::to_chars(p, buf + sizeof(buf) - 1, bc[(size_t)v], std::chars_format::general, 17).ptr
+                          : std::to_chars(p, buf + sizeof(buf) - 1, bc[(size_t)v], std::chars_format::fixed, 6).ptr;
+                *p++ = '\n';
+                s.append(buf, (size_t)(p - buf));
+            }
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                for (int64_t c = c0 + t; c < c1; c += T) fmt(c);
+            });
+        for (auto &x : th) x.join();
+        for (int64_t c = c0; c < c1; ++c) {
+            const std::string &s = out[(size_t)(c - c0)];
+            if (std::fwrite(s.data(), 1, s.size(), f) != s.size()) return false;
+        }
+    }
+    return true;
 }
 
 int main(int argc, char **argv) {
@@ -155,19 +194,6 @@ int main(int argc, char **argv) {
                     (long long)comp.arcs, nb);
         // the reference pins OpenMP to one thread after the first component
         std::printf("Num max threads %u\n", c == 0 ? (hw ? hw : 1) : 1);
-        // side-effect file: chain listing of the last component
-        // (Codes/bcc_bfs.cpp:117,145-214; truncated per component)
-        if (FILE *pp = std::fopen("PostProcessOp.txt", "w")) {
-            for (int32_t k = 0; k < comp.n_chains; ++k) {
-                int32_t p0 = comp.chain_ptr[k], cnt = comp.chain_ptr[k + 1] - p0;
-                std::fprintf(pp, "-1 %d %d %d %d\n", comp.chain_ends[2 * k], comp.chain_ends[2 * k + 1], cnt,
-                             cnt + 1);
-                for (int32_t j = 0; j < cnt; ++j)
-                    std::fprintf(pp, "%d %d %d\n", j + 1, comp.chain_nodes[p0 + j], cnt - j);
-            }
-            std::fclose(pp);
-        }
-        if (FILE *f = std::fopen("Output.txt", "w")) std::fclose(f);
         free_total += x.nf;
         if (x.nf > 0) {
             std::printf("Inside bc_gpu_free \n");
@@ -267,6 +293,23 @@ int main(int argc, char **argv) {
             scatter(x, local.data());
         }
     }
+    // side-effect files: the reference truncates and rewrites them for every
+    // component (Codes/bcc_bfs.cpp:117,145-214), so what remains is the last
+    // component's chain listing; written once here with that content
+    if (nc > 0) {
+        const bcx_component &comp = cs[nc - 1].view;
+        if (FILE *pp = std::fopen("PostProcessOp.txt", "w")) {
+            for (int32_t k = 0; k < comp.n_chains; ++k) {
+                int32_t p0 = comp.chain_ptr[k], cnt = comp.chain_ptr[k + 1] - p0;
+                std::fprintf(pp, "-1 %d %d %d %d\n", comp.chain_ends[2 * k], comp.chain_ends[2 * k + 1], cnt,
+                             cnt + 1);
+                for (int32_t j = 0; j < cnt; ++j)
+                    std::fprintf(pp, "%d %d %d\n", j + 1, comp.chain_nodes[p0 + j], cnt - j);
+            }
+            std::fclose(pp);
+        }
+        if (FILE *f = std::fopen("Output.txt", "w")) std::fclose(f);
+    }
     if (FILE *f = std::fopen("parent.txt", "w")) std::fclose(f);
     std::printf("active_vert_count %lld free_vert_count %lld \n", (long long)active_total, (long long)free_total);
     std::printf("GPU total time is %f \n", gpu_total);
@@ -275,8 +318,11 @@ int main(int argc, char **argv) {
         if (!path) continue;
         FILE *f = std::fopen(path, "w");
         if (!f) return die("write", path, 7);
-        for (int32_t v = 0; v < n; ++v) std::fprintf(f, k ? "%d %.17g\n" : "%d %f\n", v, bc[v]);
-        std::fclose(f);
+        if (!write_bc(f, bc, k == 1)) {
+            std::fclose(f);
+            return die("write", path, 7);
+        }
+        if (std::fclose(f) != 0) return die("write", path, 7);
     }
     bcx_destroy(ctx);
     bcx_host_free(h);
