@@ -198,8 +198,8 @@ static int check_simple(bcx_host *h) {
 // dfs[v] < dfs[u] and v is not the DFS parent, a component popped when
 // dfs[u] <= low[v] after returning from v (Codes/bicc.cpp:81-161).
 // The per-vertex DFS state is one 32-byte record (one cache line touched per
-// visit) and the records of the next neighbours in a list are prefetched:
-// the walk is bound by those random reads.  Returns the number of vertices
+// visit); the records of the next neighbours in a list and the adjacency of
+// the unvisited ones are prefetched: the walk is bound by those random reads.  Returns the number of vertices
 // reached from vertex 0.
 struct DfsRec {
     int32_t dfs, low, parent, pad;
@@ -207,7 +207,7 @@ struct DfsRec {
 };
 static int32_t tarjan(bcx_host *h) {
     const int32_t n = h->n;
-    constexpr int PF = 6;  // neighbours prefetched ahead of the scan
+    constexpr int PF = 4;  // prefetch distance (arcs) of the neighbour scan
     std::vector<DfsRec> rec((size_t)n);
     DfsRec *R = rec.data();
     const int32_t *adj = h->adj.data();
@@ -227,12 +227,19 @@ static int32_t tarjan(bcx_host *h) {
         R[r].dfs = R[r].low = ++last;
         R[r].parent = -1;
         stk.push_back(r);
-        for (int64_t j = R[r].it; j < std::min(R[r].end, R[r].it + PF); ++j) __builtin_prefetch(&R[adj[j]]);
+        for (int64_t j = R[r].it; j < std::min(R[r].end, R[r].it + 2 * PF); ++j) __builtin_prefetch(&R[adj[j]]);
         while (!stk.empty()) {
             const int32_t u = stk.back();
             DfsRec &U = R[u];
             if (U.it < U.end) {
-                if (U.it + PF < U.end) __builtin_prefetch(&R[adj[U.it + PF]]);
+                // records 2*PF arcs ahead; PF ahead, the record is cached, and
+                // an unvisited neighbour (a likely descent) gets its list's
+                // first line prefetched too
+                if (U.it + 2 * PF < U.end) __builtin_prefetch(&R[adj[U.it + 2 * PF]]);
+                if (U.it + PF < U.end) {
+                    const DfsRec &W = R[adj[U.it + PF]];
+                    if (W.dfs == -1) __builtin_prefetch(&adj[W.it]);
+                }
                 const int32_t v = adj[U.it++];
                 DfsRec &V = R[v];
                 if (V.dfs == -1) {
@@ -242,7 +249,7 @@ static int32_t tarjan(bcx_host *h) {
                     V.parent = u;
                     if (u == 0) children_of_0++;
                     stk.push_back(v);
-                    for (int64_t j = V.it; j < std::min(V.end, V.it + PF); ++j) __builtin_prefetch(&R[adj[j]]);
+                    for (int64_t j = V.it; j < std::min(V.end, V.it + 2 * PF); ++j) __builtin_prefetch(&R[adj[j]]);
                 } else if (V.dfs < U.dfs && v != U.parent) {
                     estk.push_back(u);
                     estk.push_back(v);
