@@ -54,11 +54,11 @@ class Stats(C.Structure):
     _fields_ = [("t_kernel_s", C.c_double), ("t_total_s", C.c_double),
                 ("n_bfs_sources", C.c_int64), ("n_chain_sources", C.c_int64),
                 ("n_row_only", C.c_int64), ("n_f64_sigma", C.c_int64), ("n_launches", C.c_int64),
-                ("max_level", C.c_int32), ("reserved", C.c_int32), ("bytes_alg", C.c_double),
+                ("max_level", C.c_int32), ("n_ball_tiles", C.c_int32), ("bytes_alg", C.c_double),
                 ("teps_work", C.c_double), ("t_chain_s", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # every symbol include/bcx.h declares, with its ctypes signature

@@ -623,12 +623,17 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         int64_t n_chain = 0;
         std::vector<int32_t> srcs = batched_sources(dc, sp, sn, o.use_chains ? &n_chain : nullptr);
         account((int64_t)srcs.size() - n_chain, n_chain);
-        if (dc.n <= SMALL_N && !(o.flags & BCX_FLAG_NO_SMALL_KERNEL))
+        if (dc.n <= SMALL_N && !(o.flags & BCX_FLAG_NO_SMALL_KERNEL)) {
             small.push_back({&dc, std::move(srcs)});
-        else if (dc.n <= WG_N && !(o.flags & BCX_FLAG_NO_WG_KERNEL) &&
-                 ((o.flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
-            wg.push_back({&dc, ball_sources(dc, srcs)});
-        else rc = batched_component(ctx, dc, srcs, &o, bc + dc.offset, s);
+            continue;
+        }
+        // larger components: tile order from the tuner (BFS ranks or balls)
+        std::vector<int32_t> tiles = tuned_sources(dc, srcs);
+        if (dc.tile_order == 1) s->n_ball_tiles++;
+        if (dc.n <= WG_N && !(o.flags & BCX_FLAG_NO_WG_KERNEL) &&
+            ((o.flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
+            wg.push_back({&dc, std::move(tiles)});
+        else rc = batched_component(ctx, dc, tiles, &o, bc + dc.offset, s);
         if (rc) return rc;
     }
     if (!small.empty() && (rc = batched_small(ctx, small, &o, bc, s))) return rc;

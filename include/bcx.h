@@ -181,7 +181,8 @@ typedef struct {
     int64_t n_f64_sigma;       /* sources redone with FP64 sigma              */
     int64_t n_launches;        /* kernel launches                             */
     int32_t max_level;         /* deepest BFS level seen                      */
-    int32_t reserved;
+    int32_t n_ball_tiles;      /* batched: components whose tiles the tile-order
+                                  tuner made graph balls (else BFS-rank runs) */
     double  bytes_alg;         /* algorithmic HBM bytes of the work done      */
     double  teps_work;         /* sum over sources of component arcs/2        */
     double  t_chain_s;         /* part of t_kernel_s spent on chain sources   */

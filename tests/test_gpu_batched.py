@@ -125,6 +125,24 @@ def test_sigma_word_escalation_through_hub_slices(engine, prec):
     assert mx < (1e-6 if prec == bcx.PREC_FP32C else 1e-9) and zeros
 
 
+def test_tile_order_tuner(engine):
+    """The tile-order tuner keeps ball tiles for a grid and BFS-rank tiles for
+    a power-law core (whichever needs fewer (vertex, level) entries on sample
+    tiles); either way the BC is the oracle's."""
+    w = 60
+    edges = [(y * w + x, y * w + x + 1) for y in range(w) for x in range(w - 1)]
+    edges += [(y * w + x, (y + 1) * w + x) for y in range(w - 1) for x in range(w)]
+    grid = comps_of(w * w, edges)[0]
+    n, uv = bcx.generate(2, 0, 0.01)
+    core = max(comps_of(n, uv), key=lambda c: c.n)
+    assert core.n > 4000
+    for c, balls, flags in ((grid, 1, 0), (core, 0, bcx.FLAG_WG_KERNEL), (core, 0, bcx.FLAG_NO_WG_KERNEL)):
+        got, st = engine.bc([c], path=B, precision=bcx.PREC_FP64, flags=flags)
+        assert st["n_ball_tiles"] == balls, (c.n, flags)
+        mx, _, zeros = relerr(got[0], oracle.component_bc(c.row_ptr, c.col, c.reach))
+        assert mx < 1e-9 and zeros
+
+
 def test_threshold_hook_and_subset(engine):
     n, uv = bcx.generate(4, 0, 0.05)
     c = max(comps_of(n, uv), key=lambda c: c.n)
