@@ -96,6 +96,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_double), _i32p]),
     "bcx_generate": (C.c_int, [C.c_int32, C.c_uint64, C.c_double, _i32p, _i64p,
                                C.POINTER(_i32p)]),
+    "bcx_batched_order": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _i32p]),
     "bcx_host_save": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "bcx_host_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
     "bcx_free": (None, [C.c_void_p]),
@@ -410,6 +411,14 @@ class DGraph:
         if rc:
             self.eng._err(rc, "run")
         return bc[:self.total], st.as_dict()
+
+    def batched_order(self, c):
+        """Local vertices of component c in the batched path's tile order."""
+        out = np.zeros(int(self.sizes[c]), np.int32)
+        rc = lib().bcx_batched_order(self.eng.h, self.g, int(c), _ptr(out))
+        if rc:
+            self.eng._err(rc, "batched_order")
+        return out
 
     def free(self):
         if self.g and self.g.value:

@@ -12,7 +12,9 @@
 // slot enqueues a tile as one fixed launch sequence and is polled through an
 // event; the host never waits on a single tile.
 #include <algorithm>
+#include <array>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "bcx_batched.h"
@@ -535,15 +537,66 @@ static int64_t tile_entries(const HostComp &hc, const int32_t *src, int ns, int6
     return entries;
 }
 
-// Tile-order tuner: on first use, BFS-rank tiles and ball tiles are
-// compared by their entries on one sample tile each (the 64 consecutive BFS
-// ranks in the middle of the order, and the ball of 64 vertices grown from
-// the first of them; the ball's count stops once it passes the ranks'):
-// balls are kept when they need at least 10% fewer.  Small-world
-// cores favour BFS ranks (2.2 against 3.1-3.8 entries per vertex per tile
-// on a 50k-vertex Chung-Lu core), grids favour balls (13.8 against 24.4 on
-// the 256x256 grid).  The choice is cached and applied to every later
-// source set of the component.
+// Landmark order: sources sorted by their distances to the LM highest-degree
+// vertices (lexicographically, BFS rank breaking ties).  Sources with equal
+// distances to every hub tend to have equal distances to everything else,
+// so a tile of them reaches a vertex at fewer distinct levels.  The LM BFS
+// passes run on host threads.
+constexpr int LM = 16;
+static std::vector<int32_t> landmark_order(const HostComp &hc, const std::vector<int32_t> &rank) {
+    const int32_t n = hc.n;
+    const int K = std::min<int32_t>(LM, n);
+    std::vector<int32_t> hubs((size_t)n);
+    for (int32_t v = 0; v < n; ++v) hubs[v] = v;
+    std::partial_sort(hubs.begin(), hubs.begin() + K, hubs.end(), [&](int32_t a, int32_t b) {
+        const int32_t da = hc.rp[a + 1] - hc.rp[a], db = hc.rp[b + 1] - hc.rp[b];
+        return da != db ? da > db : a < b;
+    });
+    // dist[v][k], saturated at 65535
+    std::vector<std::array<uint16_t, LM>> dist((size_t)n);
+    auto bfs = [&](int k) {
+        std::vector<int32_t> d((size_t)n, -1), q;
+        q.reserve(n);
+        q.push_back(hubs[k]);
+        d[hubs[k]] = 0;
+        for (size_t i = 0; i < q.size(); ++i) {
+            const int32_t v = q[i];
+            for (int32_t j = hc.rp[v]; j < hc.rp[v + 1]; ++j)
+                if (d[hc.col[j]] < 0) {
+                    d[hc.col[j]] = d[v] + 1;
+                    q.push_back(hc.col[j]);
+                }
+        }
+        for (int32_t v = 0; v < n; ++v) dist[v][k] = (uint16_t)std::min(d[v] < 0 ? 65535 : d[v], 65535);
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < hw; ++t)
+        th.emplace_back([&, t] {
+            for (int k = (int)t; k < K; k += (int)hw) bfs(k);
+        });
+    for (auto &x : th) x.join();
+    for (int32_t v = 0; v < n; ++v)
+        for (int k = K; k < LM; ++k) dist[v][k] = 0;
+    std::vector<int32_t> order((size_t)n);
+    for (int32_t v = 0; v < n; ++v) order[(size_t)rank[v]] = v;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return dist[a] < dist[b]; });  // ties keep BFS rank
+    return order;
+}
+
+// Tile-order tuner: on first use, three orders of all the component's
+// vertices are compared by the entries of one sample tile each: BFS rank
+// (the 64 consecutive ranks in the middle), a ball (the 64 vertices nearest
+// to that tile's first source) and the landmark order (its middle tile).  A
+// ball is kept when it needs at least 10% fewer entries than BFS rank (ball
+// tiles are built per source set and cost host time), the landmark order
+// when it needs at least 2% fewer than the better of the two.  Measured
+// entries per vertex per tile: small-world cores 2.2 (BFS rank), 3.1-3.8
+// (balls), 2.1 (landmarks) on a 50k-vertex Chung-Lu core; 1.97 / 1.81 (BFS
+// rank / landmarks) on an R-MAT core; grids 24.4 (BFS rank) against 13.8
+// (balls).  The choice is cached and applied to every later source set of
+// the component.
 std::vector<int32_t> tuned_sources(DComp &dc, const std::vector<int32_t> &srcs) {
     if (dc.tile_order < 0) {
         const HostComp &hc = dc.host;
@@ -555,27 +608,44 @@ std::vector<int32_t> tuned_sources(DComp &dc, const std::vector<int32_t> &srcs) 
             std::vector<int32_t> by_rank((size_t)n);
             for (int32_t v = 0; v < n; ++v) by_rank[(size_t)dc.rank[v]] = v;
             std::vector<uint64_t> vis((size_t)n, 0), cur((size_t)n, 0), nxt((size_t)n, 0);
-            std::vector<int32_t> stamp((size_t)n, -1), q, ball;
-            int64_t e_rank = 0, e_ball = 0;
-            {
-                const int64_t t = T / 2;
-                const int32_t *tile = by_rank.data() + t * BATCH_W;
-                e_rank += tile_entries(hc, tile, BATCH_W, INT64_MAX, vis, cur, nxt);
-                // the 64 vertices nearest to the tile's first source
-                q.assign(1, tile[0]);
-                stamp[tile[0]] = (int32_t)t;
-                for (size_t i = 0; i < q.size() && q.size() < (size_t)BATCH_W; ++i)
-                    for (int32_t j = hc.rp[q[i]]; j < hc.rp[q[i] + 1] && q.size() < (size_t)BATCH_W; ++j)
-                        if (stamp[hc.col[j]] != (int32_t)t) {
-                            stamp[hc.col[j]] = (int32_t)t;
-                            q.push_back(hc.col[j]);
-                        }
-                e_ball += tile_entries(hc, q.data(), (int)q.size(), e_rank - e_ball, vis, cur, nxt);
+            std::vector<int32_t> stamp((size_t)n, -1), q;
+            const int64_t t = T / 2;
+            const int32_t *tile = by_rank.data() + t * BATCH_W;
+            const int64_t e_rank = tile_entries(hc, tile, BATCH_W, INT64_MAX, vis, cur, nxt);
+            // the 64 vertices nearest to the tile's first source
+            q.assign(1, tile[0]);
+            stamp[tile[0]] = 0;
+            for (size_t i = 0; i < q.size() && q.size() < (size_t)BATCH_W; ++i)
+                for (int32_t j = hc.rp[q[i]]; j < hc.rp[q[i] + 1] && q.size() < (size_t)BATCH_W; ++j)
+                    if (stamp[hc.col[j]] != 0) {
+                        stamp[hc.col[j]] = 0;
+                        q.push_back(hc.col[j]);
+                    }
+            const int64_t e_ball = tile_entries(hc, q.data(), (int)q.size(), e_rank, vis, cur, nxt);
+            const std::vector<int32_t> by_lm = landmark_order(hc, dc.rank);
+            const int64_t best = std::min(e_rank, 10 * e_ball <= 9 * e_rank ? e_ball : e_rank);
+            const int64_t e_lm = tile_entries(hc, by_lm.data() + t * BATCH_W, BATCH_W, best, vis, cur, nxt);
+            if (50 * e_lm <= 49 * best) {
+                dc.tile_order = 2;
+                dc.lm_pos.assign((size_t)n, 0);
+                for (int32_t i = 0; i < n; ++i) dc.lm_pos[(size_t)by_lm[i]] = i;
+            } else if (10 * e_ball <= 9 * e_rank) {
+                dc.tile_order = 1;
             }
-            dc.tile_order = (10 * e_ball <= 9 * e_rank) ? 1 : 0;
         }
     }
-    return dc.tile_order == 1 ? ball_sources(dc, srcs) : srcs;
+    if (dc.tile_order == 1) return ball_sources(dc, srcs);
+    if (dc.tile_order == 2) {
+        std::vector<int32_t> o(srcs);
+        std::sort(o.begin(), o.end(), [&](int32_t a, int32_t b) { return dc.lm_pos[a] < dc.lm_pos[b]; });
+        return o;
+    }
+    return srcs;
+}
+
+std::vector<int32_t> batched_order(DComp &dc) {
+    std::vector<int32_t> all = batched_sources(dc, nullptr, 0, nullptr);
+    return dc.n <= SMALL_N ? all : tuned_sources(dc, all);
 }
 
 // The component relabelled by descending degree (stable): the hubs' rows,

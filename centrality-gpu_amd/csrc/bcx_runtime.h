@@ -50,7 +50,8 @@ struct DComp {
     std::vector<int32_t> rank;      // batched path: BFS rank of each vertex (built on first use)
     std::vector<uint8_t> on_chain;  // batched path: 1 for chain vertices (built on first use)
     std::vector<int32_t> ball;      // batched path: every vertex in ball-tile order (built on first use)
-    int8_t tile_order = -1;         // batched path: 0 BFS-rank tiles, 1 ball tiles (tuned on first use)
+    int8_t tile_order = -1;         // batched path: 0 BFS-rank, 1 ball, 2 landmark tiles (tuned on first use)
+    std::vector<int32_t> lm_pos;    // batched path: position of each vertex in the landmark order
     HostComp host;
     DBuf b_rp, b_col, b_reach, b_chp, b_chn, b_che;
     CompDev dev{};
@@ -131,6 +132,9 @@ struct SmallJob {
 std::vector<int32_t> batched_sources(bcx::DComp &dc, const int32_t *subset, int64_t n_sub, int64_t *n_chain);
 std::vector<int32_t> ball_sources(bcx::DComp &dc, const std::vector<int32_t> &srcs);
 std::vector<int32_t> tuned_sources(bcx::DComp &dc, const std::vector<int32_t> &srcs);
+// Every vertex of the component in the order a full batched run tiles its
+// sources (64 consecutive entries per tile).
+std::vector<int32_t> batched_order(bcx::DComp &dc);
 int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
                   bcx_stats *st);
 int batched_wg(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,

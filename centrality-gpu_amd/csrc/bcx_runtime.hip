@@ -551,6 +551,13 @@ int bcx_upload(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps, bcx_dg
     return BCX_OK;
 }
 
+int bcx_batched_order(bcx_ctx *ctx, bcx_dgraph *g, int32_t c, int32_t *order) {
+    if (!ctx || !g || !order || c < 0 || c >= (int32_t)g->comps.size()) return BCX_E_ARG;
+    const std::vector<int32_t> o = batched_order(g->comps[c]);
+    std::memcpy(order, o.data(), sizeof(int32_t) * o.size());
+    return BCX_OK;
+}
+
 void bcx_dgraph_free(bcx_ctx *ctx, bcx_dgraph *g) {
     if (!g) return;
     if (ctx) {

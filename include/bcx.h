@@ -203,6 +203,13 @@ int  bcx_upload(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps,
                 bcx_dgraph **out);
 void bcx_dgraph_free(bcx_ctx *ctx, bcx_dgraph *g);
 
+/* Batched path: every local vertex of component c of g in the order a run
+ * with default flags tiles its sources (64 consecutive entries per tile; a
+ * run over a subset keeps this order).  Runs the per-component tile-order
+ * tuner on first use.  No counterpart in the reference (its sources are one
+ * per block in id order); for harnesses that time samples of whole tiles.  */
+int  bcx_batched_order(bcx_ctx *ctx, bcx_dgraph *g, int32_t c, int32_t *order);
+
 /* Weighted Brandes over every component of g (begin_gpu free + active +
  * postprocess, Codes/begin_gpupgm.cu:42-89).  bc_out is a host array of
  * sum(n_c) doubles, component c at offset sum_{c'<c} n_{c'}; results are

@@ -143,6 +143,32 @@ def test_tile_order_tuner(engine):
         assert mx < 1e-9 and zeros
 
 
+def test_batched_order_whole_tiles(engine):
+    """bcx_batched_order: every component's vertices in the tuned tile order
+    (a permutation); a subset made of whole tiles of that order gives the
+    oracle's partial BC for those sources."""
+    n, uv = bcx.generate(2, 0, 0.03)
+    comps = comps_of(n, uv)
+    g = engine.upload(comps)
+    try:
+        for ci, c in enumerate(comps):
+            o = g.batched_order(ci)
+            assert np.array_equal(np.sort(o), np.arange(c.n)), ci
+        ci = int(np.argmax([c.n for c in comps]))
+        c = comps[ci]
+        o = g.batched_order(ci)
+        T = c.n // 64
+        tiles = np.concatenate([o[t * 64:(t + 1) * 64] for t in (0, T // 2, T - 1)])
+        sub = np.sort(tiles).astype(np.int64) + g.offsets[ci]
+        want = oracle.component_bc(c.row_ptr, c.col, c.reach, sources=np.sort(tiles).astype(np.int32))
+        for prec, tol in ((bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)):
+            got, _ = g.run(path=B, precision=prec, subset=sub)
+            mx, _, zeros = relerr(got[g.offsets[ci]:g.offsets[ci] + c.n], want)
+            assert mx < tol and zeros, prec
+    finally:
+        g.free()
+
+
 def test_threshold_hook_and_subset(engine):
     n, uv = bcx.generate(4, 0, 0.05)
     c = max(comps_of(n, uv), key=lambda c: c.n)
