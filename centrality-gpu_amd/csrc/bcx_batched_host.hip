@@ -495,10 +495,10 @@ std::vector<int32_t> ball_sources(DComp &dc, const std::vector<int32_t> &srcs) {
     return out;
 }
 
-// (vertex, level) entries of one tile of ns <= 64 sources: a host 64-lane
-// level-synchronous BFS (the tile kernels' own scheme, on the CPU).  The
-// batched kernels' work per tile is proportional to these entries and the
-// arcs they walk.  Stops early once the count passes `limit`.
+// Work of one tile of ns <= 64 sources: its (vertex, level) entries, each
+// weighted by the arcs the pull and the backward walk for it (degree + 1),
+// from a host 64-lane level-synchronous BFS (the tile kernels' own scheme,
+// on the CPU).  Stops early once the count passes `limit`.
 static int64_t tile_entries(const HostComp &hc, const int32_t *src, int ns, int64_t limit,
                             std::vector<uint64_t> &vis, std::vector<uint64_t> &cur, std::vector<uint64_t> &nxt) {
     std::vector<int32_t> fr, nf, seen;
@@ -509,7 +509,8 @@ static int64_t tile_entries(const HostComp &hc, const int32_t *src, int ns, int6
         vis[v] |= 1ull << k;
     }
     seen = fr;
-    int64_t entries = (int64_t)fr.size();
+    int64_t entries = 0;
+    for (int32_t v : fr) entries += hc.rp[v + 1] - hc.rp[v] + 1;
     while (!fr.empty() && entries <= limit) {
         nf.clear();
         for (int32_t v : fr) {
@@ -528,8 +529,8 @@ static int64_t tile_entries(const HostComp &hc, const int32_t *src, int ns, int6
             vis[w] |= nxt[w];
             cur[w] = nxt[w];
             nxt[w] = 0;
+            entries += hc.rp[w + 1] - hc.rp[w] + 1;
         }
-        entries += (int64_t)nf.size();
         fr.swap(nf);
     }
     for (int32_t v : seen) vis[v] = 0;
@@ -585,32 +586,55 @@ static std::vector<int32_t> landmark_order(const HostComp &hc, const std::vector
     return order;
 }
 
+// Work of the tiles of `order` at nine evenly spaced positions, on host
+// threads (the work of single tiles varies by tens of percent).
+static int64_t sample_work(const HostComp &hc, const std::vector<int32_t> &order) {
+    const int64_t T = ((int64_t)hc.n + BATCH_W - 1) / BATCH_W;
+    constexpr int S = 9;
+    std::vector<int64_t> w(S, 0);
+    const unsigned nt = std::max(1u, std::min<unsigned>(S, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            std::vector<uint64_t> vis((size_t)hc.n, 0), cur((size_t)hc.n, 0), nxt((size_t)hc.n, 0);
+            for (int k = (int)t; k < S; k += (int)nt) {
+                const int64_t u = (k + 1) * T / (S + 1);
+                const int ns = (int)std::min<int64_t>(BATCH_W, hc.n - u * BATCH_W);
+                w[k] = tile_entries(hc, order.data() + u * BATCH_W, ns, INT64_MAX, vis, cur, nxt);
+            }
+        });
+    for (auto &x : th) x.join();
+    int64_t sum = 0;
+    for (int64_t x : w) sum += x;
+    return sum;
+}
+
 // Tile-order tuner: on first use, three orders of all the component's
-// vertices are compared by the entries of one sample tile each: BFS rank
-// (the 64 consecutive ranks in the middle), a ball (the 64 vertices nearest
-// to that tile's first source) and the landmark order (its middle tile).  A
-// ball is kept when it needs at least 10% fewer entries than BFS rank (ball
-// tiles are built per source set and cost host time), the landmark order
-// when it needs at least 2% fewer than the better of the two.  Measured
-// entries per vertex per tile: small-world cores 2.2 (BFS rank), 3.1-3.8
-// (balls), 2.1 (landmarks) on a 50k-vertex Chung-Lu core; 1.97 / 1.81 (BFS
-// rank / landmarks) on an R-MAT core; grids 24.4 (BFS rank) against 13.8
-// (balls).  The choice is cached and applied to every later source set of
-// the component.
+// vertices are compared by the work of sample tiles (see tile_entries).
+// BFS rank against a ball: the 64 consecutive ranks in the middle of the
+// order against the 64 vertices nearest to that tile's first source; the
+// ball order is kept when it needs at least 10% less work (ball tiles are
+// built per source set and cost host time; on the 256x256 grid balls need
+// 13.8 entries per vertex per tile against 24.4).  Otherwise BFS rank
+// against the landmark order, on nine tiles of each: landmarks are kept when
+// they need at least 2% less.  Arcs walked per arc per tile on nine tiles
+// (BFS rank / landmarks): 1.98 / 1.89 on the 835k-vertex core of the
+// 1M-vertex mixed graph, 1.68 / 1.61 on an R-MAT core with chains, 1.82 /
+// 1.88 on a 519k-vertex Chung-Lu core without chains.  The choice is cached
+// and applied to every later source set of the component.
 std::vector<int32_t> tuned_sources(DComp &dc, const std::vector<int32_t> &srcs) {
     if (dc.tile_order < 0) {
         const HostComp &hc = dc.host;
         const int32_t n = hc.n;
         const int64_t T = ((int64_t)n + BATCH_W - 1) / BATCH_W;
         dc.tile_order = 0;
-        if (T >= 8) {
+        if (T >= 16) {
             if (dc.rank.empty()) dc.rank = bfs_rank(hc);
             std::vector<int32_t> by_rank((size_t)n);
             for (int32_t v = 0; v < n; ++v) by_rank[(size_t)dc.rank[v]] = v;
             std::vector<uint64_t> vis((size_t)n, 0), cur((size_t)n, 0), nxt((size_t)n, 0);
             std::vector<int32_t> stamp((size_t)n, -1), q;
-            const int64_t t = T / 2;
-            const int32_t *tile = by_rank.data() + t * BATCH_W;
+            const int32_t *tile = by_rank.data() + (T / 2) * BATCH_W;
             const int64_t e_rank = tile_entries(hc, tile, BATCH_W, INT64_MAX, vis, cur, nxt);
             // the 64 vertices nearest to the tile's first source
             q.assign(1, tile[0]);
@@ -622,15 +646,15 @@ std::vector<int32_t> tuned_sources(DComp &dc, const std::vector<int32_t> &srcs) 
                         q.push_back(hc.col[j]);
                     }
             const int64_t e_ball = tile_entries(hc, q.data(), (int)q.size(), e_rank, vis, cur, nxt);
-            const std::vector<int32_t> by_lm = landmark_order(hc, dc.rank);
-            const int64_t best = std::min(e_rank, 10 * e_ball <= 9 * e_rank ? e_ball : e_rank);
-            const int64_t e_lm = tile_entries(hc, by_lm.data() + t * BATCH_W, BATCH_W, best, vis, cur, nxt);
-            if (50 * e_lm <= 49 * best) {
-                dc.tile_order = 2;
-                dc.lm_pos.assign((size_t)n, 0);
-                for (int32_t i = 0; i < n; ++i) dc.lm_pos[(size_t)by_lm[i]] = i;
-            } else if (10 * e_ball <= 9 * e_rank) {
+            if (10 * e_ball <= 9 * e_rank) {
                 dc.tile_order = 1;
+            } else {
+                const std::vector<int32_t> by_lm = landmark_order(hc, dc.rank);
+                if (50 * sample_work(hc, by_lm) <= 49 * sample_work(hc, by_rank)) {
+                    dc.tile_order = 2;
+                    dc.lm_pos.assign((size_t)n, 0);
+                    for (int32_t i = 0; i < n; ++i) dc.lm_pos[(size_t)by_lm[i]] = i;
+                }
             }
         }
     }
