@@ -1,6 +1,6 @@
 """Writes tests/golden/goldens.json: expected outputs computed HERE with
-networkx 3.4.2 (run once in the build container; the GPU box never needs
-networkx).  Inputs: the reference's own sample inputs
+networkx 3.4.2 (run once where networkx is installed; the tests only read
+the JSON, so machines without networkx can run them).  Inputs: the reference's own sample inputs
 (Codes/sample_input/test1..4.txt, copied beside this script as data) and
 small seeded graphs with closed forms or known structure.
 
