@@ -747,6 +747,13 @@ int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs,
 struct SmallState {
     DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs;
     int grid = 0;
+    // a pass started and not yet finished (persistent_start / _finish)
+    bool pending = false, run_fp32 = false, run_graph = false;
+    SmallArgs run{};
+    std::vector<SmallTile> run_tiles;
+    size_t run_narrow = 0;
+    int run_grid = 0;
+    hipStream_t run_stream = nullptr;
     // captured launch (counter reset kernel + persistent kernel), replayed
     // while its arguments stay the same
     hipGraphExec_t exec = nullptr;
@@ -757,10 +764,10 @@ struct SmallState {
 
 // Reset + persistent kernel, replayed from a captured graph when possible.
 static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool f64, bool fp32, int grid,
-                        bool use_graph) {
+                        bool use_graph, hipStream_t sp) {
     if (!use_graph) {
-        int rc = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "small reset");
-        return rc ? rc : ctx_hip(ctx, launch_small(a, f64, fp32, grid, ctx->stream), "small");
+        int rc = ctx_hip(ctx, launch_small_reset(a.counter, sp), "small reset");
+        return rc ? rc : ctx_hip(ctx, launch_small(a, f64, fp32, grid, sp), "small");
     }
     const bool same = ss.exec && !std::memcmp(&ss.key, &a, sizeof(a)) && ss.key_grid == grid &&
                       ss.key_f64 == f64 && ss.key_fp32 == fp32;
@@ -768,11 +775,11 @@ static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool f
         if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
         ss.exec = nullptr;
         hipGraph_t graph = nullptr;
-        int rc = ctx_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "capture");
+        int rc = ctx_hip(ctx, hipStreamBeginCapture(sp, hipStreamCaptureModeThreadLocal), "capture");
         if (rc) return rc;
-        hipError_t e1 = launch_small_reset(a.counter, ctx->stream);
-        hipError_t e2 = launch_small(a, f64, fp32, grid, ctx->stream);
-        hipError_t e3 = hipStreamEndCapture(ctx->stream, &graph);
+        hipError_t e1 = launch_small_reset(a.counter, sp);
+        hipError_t e2 = launch_small(a, f64, fp32, grid, sp);
+        hipError_t e3 = hipStreamEndCapture(sp, &graph);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
             return ctx_hip(ctx, e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "capture");
@@ -785,15 +792,37 @@ static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool f
         ss.key_f64 = f64;
         ss.key_fp32 = fp32;
     }
-    return ctx_hip(ctx, hipGraphLaunch(ss.exec, ctx->stream), "graph launch");
+    return ctx_hip(ctx, hipGraphLaunch(ss.exec, sp), "graph launch");
+}
+
+// Orders stream `sp` after everything enqueued so far on the context stream
+// (uploads of the pass's tables).
+static int follow_ctx_stream(bcx_ctx *ctx, hipStream_t sp) {
+    if (sp == ctx->stream) return BCX_OK;
+    hipEvent_t ev;
+    int rc = ctx_hip(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+    if (rc) return rc;
+    rc = ctx_hip(ctx, hipEventRecord(ev, ctx->stream), "event");
+    if (!rc) rc = ctx_hip(ctx, hipStreamWaitEvent(sp, ev, 0), "event");
+    (void)hipEventDestroy(ev);
+    return rc;
+}
+
+static int persistent_pass(bcx_ctx *ctx, SmallState &ss, bool wg, bool f64, bool graph) {
+    const SmallArgs &a = ss.run;
+    if (!wg) return small_launch(ctx, ss, a, f64, ss.run_fp32, ss.run_grid, graph, ss.run_stream);
+    int r = ctx_hip(ctx, launch_small_reset(a.counter, ss.run_stream), "workgroup reset");
+    return r ? r : ctx_hip(ctx, launch_wg(a, f64, ss.run_fp32, ss.run_grid, ss.run_stream), "workgroup kernel");
 }
 
 // One persistent launch over the tiles of `jobs`: the small-component kernel
 // (LDS state, n <= SMALL_N) or, with `wg`, the workgroup-per-tile kernel
 // (HBM slabs, n <= WG_N).  Tiles whose uint64 sigma overflows are rerun in
 // FP64 by a second launch.
-static int persistent_tiles(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
-                            bcx_stats *st, bool wg) {
+// Starts the pass on stream sp and returns without waiting for it;
+// persistent_finish waits, reruns the overflowed tiles and completes it.
+static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+                            bcx_stats *st, bool wg, hipStream_t sp) {
     if (jobs.empty()) return BCX_OK;
     if (!ctx->batched) ctx->batched = new BatchedState();
     SmallState *&state = wg ? ctx->wgk : ctx->small;
@@ -884,61 +913,77 @@ static int persistent_tiles(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     a.qoffs = wg ? (int32_t *)ss.qoffs.p : nullptr;
     a.nmax = nmax;
     a.qcap = (int64_t)qcap;
-    const bool fp32 = o->precision == BCX_PREC_FP32C;
-    const bool use_graph = !wg && !(o->flags & BCX_FLAG_NO_GRAPH);
-    auto launch = [&](bool f64, bool graph) -> int {
-        if (!wg) return small_launch(ctx, ss, a, f64, fp32, (int)grid, graph);
-        int r = ctx_hip(ctx, launch_small_reset(a.counter, ctx->stream), "workgroup reset");
-        return r ? r : ctx_hip(ctx, launch_wg(a, f64, fp32, (int)grid, ctx->stream), "workgroup kernel");
-    };
+    ss.run = a;
+    ss.run_fp32 = o->precision == BCX_PREC_FP32C;
+    ss.run_graph = !wg && !(o->flags & BCX_FLAG_NO_GRAPH);
+    ss.run_tiles = std::move(tiles);
+    ss.run_narrow = n_narrow_tiles;
+    ss.run_grid = (int)grid;
+    ss.run_stream = sp;
+    ss.pending = true;
     // uint64 pass over the tiles not predicted to overflow
-    int32_t h[2] = {0, 0};
-    a.n_tiles = (int32_t)n_narrow_tiles;
-    if (a.n_tiles > 0) {
+    ss.run.n_tiles = (int32_t)n_narrow_tiles;
+    if (ss.run.n_tiles > 0) {
+        if ((rc = follow_ctx_stream(ctx, sp))) return rc;
         ctx->timer.begin(ctx);
-        if ((rc = launch(false, use_graph))) return rc;
+        if ((rc = persistent_pass(ctx, ss, wg, false, ss.run_graph))) return rc;
         ctx->timer.end(ctx);
         if (st) st->n_launches += 2;
-        if ((rc = ctx_hip(ctx, hipMemcpyAsync(h, a.counter, sizeof(h), hipMemcpyDeviceToHost, ctx->stream),
-                          "counter")) ||
-            (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel")))
-            return rc;
-    }
-    if (h[1] > 0 || n_narrow_tiles < tiles.size()) {
-        // tiles whose uint64 sigma overflowed added nothing: rerun them in
-        // FP64, with the tiles that start there
-        std::vector<int32_t> redo((size_t)h[1]);
-        if (h[1] > 0 &&
-            ((rc = ctx_hip(ctx, hipMemcpyAsync(redo.data(), a.redo, 4 * redo.size(), hipMemcpyDeviceToHost,
-                                               ctx->stream), "redo")) ||
-             (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "redo"))))
-            return rc;
-        std::vector<SmallTile> again;
-        for (int32_t t : redo) again.push_back(tiles[t]);
-        for (size_t t = n_narrow_tiles; t < tiles.size(); ++t) again.push_back(tiles[t]);
-        std::sort(again.begin(), again.end(), [](const SmallTile &x, const SmallTile &y) { return x.src0 < y.src0; });
-        if ((rc = upload(ctx, again.data(), sizeof(SmallTile) * again.size(), ss.tiles, "small tiles"))) return rc;
-        a.tiles = (const SmallTile *)ss.tiles.p;
-        a.n_tiles = (int32_t)again.size();
-        a.limit = 0;
-        if (st) for (auto &t : again) st->n_f64_sigma += t.ns;
-        ctx->timer.begin(ctx);
-        if ((rc = launch(true, false))) return rc;
-        ctx->timer.end(ctx);
-        if (st) st->n_launches += 2;
-        if ((rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "small kernel"))) return rc;
     }
     return BCX_OK;
 }
 
-int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
-                  bcx_stats *st) {
-    return persistent_tiles(ctx, jobs, o, bc_base, st, false);
+static int persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg) {
+    SmallState *ps = wg ? ctx->wgk : ctx->small;
+    if (!ps || !ps->pending) return BCX_OK;
+    SmallState &ss = *ps;
+    ss.pending = false;
+    const hipStream_t sp = ss.run_stream;
+    int rc;
+    int32_t h[2] = {0, 0};
+    if (ss.run.n_tiles > 0 &&
+        ((rc = ctx_hip(ctx, hipMemcpyAsync(h, ss.run.counter, sizeof(h), hipMemcpyDeviceToHost, sp), "counter")) ||
+         (rc = ctx_hip(ctx, hipStreamSynchronize(sp), "persistent kernel"))))
+        return rc;
+    if (h[1] > 0 || ss.run_narrow < ss.run_tiles.size()) {
+        // tiles whose uint64 sigma overflowed added nothing: rerun them in
+        // FP64, with the tiles that start there
+        std::vector<int32_t> redo((size_t)h[1]);
+        if (h[1] > 0 &&
+            ((rc = ctx_hip(ctx, hipMemcpyAsync(redo.data(), ss.run.redo, 4 * redo.size(), hipMemcpyDeviceToHost, sp),
+                           "redo")) ||
+             (rc = ctx_hip(ctx, hipStreamSynchronize(sp), "redo"))))
+            return rc;
+        std::vector<SmallTile> again;
+        for (int32_t t : redo) again.push_back(ss.run_tiles[t]);
+        for (size_t t = ss.run_narrow; t < ss.run_tiles.size(); ++t) again.push_back(ss.run_tiles[t]);
+        std::sort(again.begin(), again.end(), [](const SmallTile &x, const SmallTile &y) { return x.src0 < y.src0; });
+        if ((rc = upload(ctx, again.data(), sizeof(SmallTile) * again.size(), ss.tiles, "small tiles"))) return rc;
+        ss.run.tiles = (const SmallTile *)ss.tiles.p;
+        ss.run.n_tiles = (int32_t)again.size();
+        ss.run.limit = 0;
+        if (st) for (auto &t : again) st->n_f64_sigma += t.ns;
+        if ((rc = follow_ctx_stream(ctx, sp))) return rc;
+        ctx->timer.begin(ctx);
+        if ((rc = persistent_pass(ctx, ss, wg, true, false))) return rc;
+        ctx->timer.end(ctx);
+        if (st) st->n_launches += 2;
+        if ((rc = ctx_hip(ctx, hipStreamSynchronize(sp), "persistent kernel"))) return rc;
+    }
+    return BCX_OK;
 }
 
-int batched_wg(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
-               bcx_stats *st) {
-    return persistent_tiles(ctx, jobs, o, bc_base, st, true);
+int batched_persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+                             bcx_stats *st, bool wg, hipStream_t sp) {
+    return persistent_start(ctx, jobs, o, bc_base, st, wg, sp);
+}
+
+int batched_persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg) { return persistent_finish(ctx, st, wg); }
+
+void batched_persistent_reset(bcx_ctx *ctx) {
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->small) ctx->small->pending = false;
+    if (ctx->wgk) ctx->wgk->pending = false;
 }
 
 static void release_state(bcx_ctx *ctx, SmallState *&state) {

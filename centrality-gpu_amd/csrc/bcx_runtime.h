@@ -23,9 +23,13 @@ struct DBuf {
     void release(bcx_ctx *ctx);
 };
 
+// HIP-event intervals on the context stream, summed by collect().  Nested
+// begin/end pairs record only at the outermost level, so a run whose
+// streams overlap is timed as one interval.
 struct Timer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
     size_t used = 0;
+    int depth = 0;
     void begin(bcx_ctx *ctx);
     void end(bcx_ctx *ctx);
     double collect();
@@ -93,6 +97,7 @@ struct bcx_ctx {
     size_t mem_cap = 0, mem_used = 0;
     std::map<void *, size_t> sizes;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // batched: persistent passes beside the multi-launch tiles
     std::string err;
     bool overflow = false;
     int plain_wg_per_cu = 2;
@@ -135,10 +140,14 @@ std::vector<int32_t> tuned_sources(bcx::DComp &dc, const std::vector<int32_t> &s
 // Every vertex of the component in the order a full batched run tiles its
 // sources (64 consecutive entries per tile).
 std::vector<int32_t> batched_order(bcx::DComp &dc);
-int batched_small(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
-                  bcx_stats *st);
-int batched_wg(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
-               bcx_stats *st);
+// One persistent launch over the tiles of `jobs` (the small-component
+// kernel, or with `wg` the workgroup-per-tile kernel), started on stream sp
+// without waiting; _finish waits for it and reruns overflowed tiles in FP64.
+int batched_persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
+                             bcx_stats *st, bool wg, hipStream_t sp);
+int batched_persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg);
+// Drops passes left unfinished by an earlier failed run (after waiting for them).
+void batched_persistent_reset(bcx_ctx *ctx);
 bool batched_narrow(bcx::DComp &dc);
 void small_release(bcx_ctx *ctx);
 int batched_component(bcx_ctx *ctx, bcx::DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o,

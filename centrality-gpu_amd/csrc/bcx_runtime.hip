@@ -99,6 +99,7 @@ int upload(bcx_ctx *ctx, const void *host, size_t bytes, DBuf &buf, const char *
 
 // Kernel time: events around every launch, summed after the final sync.
 void Timer::begin(bcx_ctx *ctx) {
+    if (depth++ > 0) return;
     if (used == evs.size()) {
         hipEvent_t a, b;
         (void)hipEventCreate(&a);
@@ -108,6 +109,7 @@ void Timer::begin(bcx_ctx *ctx) {
     (void)hipEventRecord(evs[used].first, ctx->stream);
 }
 void Timer::end(bcx_ctx *ctx) {
+    if (--depth > 0) return;
     (void)hipEventRecord(evs[used].second, ctx->stream);
     used++;
 }
@@ -457,6 +459,7 @@ void bcx_destroy(bcx_ctx *ctx) {
     small_release(ctx);
     ctx->timer.destroy();
     ctx->timer_chain.destroy();
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -603,6 +606,7 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
     }
     std::vector<SmallJob> small;  // batched path: components for the one-launch small kernel
     std::vector<SmallJob> wg;     // batched path: deep components for the workgroup-per-tile kernel
+    std::vector<SmallJob> multi;  // batched path: components for the multi-launch tiles
     for (size_t ci = 0; ci < g->comps.size(); ++ci) {
         DComp &dc = g->comps[ci];
         const int32_t *sp = o.src_subset ? sub[ci].data() : nullptr;
@@ -640,11 +644,37 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         if (dc.n <= WG_N && !(o.flags & BCX_FLAG_NO_WG_KERNEL) &&
             ((o.flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
             wg.push_back({&dc, std::move(tiles)});
-        else rc = batched_component(ctx, dc, tiles, &o, bc + dc.offset, s);
-        if (rc) return rc;
+        else multi.push_back({&dc, std::move(tiles)});
     }
-    if (!small.empty() && (rc = batched_small(ctx, small, &o, bc, s))) return rc;
-    if (!wg.empty() && (rc = batched_wg(ctx, wg, &o, bc, s))) return rc;
+    if (o.path == BCX_PATH_BATCHED && (!small.empty() || !wg.empty() || !multi.empty())) {
+        // the persistent passes (small components, deep components) run on a
+        // side stream while the multi-launch tiles run on theirs; the run is
+        // timed as one interval on the context stream around both
+        const bool overlap = !multi.empty() && (!small.empty() || !wg.empty());
+        if (overlap && !ctx->side &&
+            (rc = ctx_hip(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "side stream")))
+            return rc;
+        const hipStream_t sp = overlap ? ctx->side : ctx->stream;
+        batched_persistent_reset(ctx);
+        ctx->timer.depth = 0;
+        ctx->timer.begin(ctx);
+        if (!small.empty() && (rc = batched_persistent_start(ctx, small, &o, bc, s, false, sp))) return rc;
+        if (!wg.empty() && (rc = batched_persistent_start(ctx, wg, &o, bc, s, true, sp))) return rc;
+        for (SmallJob &jb : multi)
+            if ((rc = batched_component(ctx, *const_cast<DComp *>(jb.dc), jb.srcs, &o, bc + jb.dc->offset, s)))
+                return rc;
+        if ((rc = batched_persistent_finish(ctx, s, false)) || (rc = batched_persistent_finish(ctx, s, true)))
+            return rc;
+        if (overlap) {
+            hipEvent_t ev;
+            if ((rc = ctx_hip(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event"))) return rc;
+            rc = ctx_hip(ctx, hipEventRecord(ev, sp), "event");
+            if (!rc) rc = ctx_hip(ctx, hipStreamWaitEvent(ctx->stream, ev, 0), "event");
+            (void)hipEventDestroy(ev);
+            if (rc) return rc;
+        }
+        ctx->timer.end(ctx);
+    }
     rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "run");
     if (rc) return rc;
     s->t_chain_s = ctx->timer_chain.collect();
