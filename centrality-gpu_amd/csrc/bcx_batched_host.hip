@@ -672,19 +672,19 @@ std::vector<int32_t> batched_order(DComp &dc) {
     return dc.n <= SMALL_N ? all : tuned_sources(dc, all);
 }
 
-// The component relabelled by descending degree (stable): the hubs' rows,
-// masks and sigma / term rows, which most arcs point at, become one
-// contiguous, cache-resident block.  Each row keeps its arcs in the original
-// order, so the FP64 sigma sums keep the oracle's order.
+// The component relabelled in BFS order from its highest-degree vertex: the
+// vertices a tile reaches at one level lie in a few consecutive BFS layers
+// of that vertex (a tile's sources are close to each other), so each level's
+// masks, sigma / term rows and list entries fall in a compact id range, and
+// the hub itself and its neighbours lead the arrays.  Each row keeps its arcs
+// in the original order, so the FP64 sigma sums keep the oracle's order.
 static int relabelled(bcx_ctx *ctx, DComp &dc, DComp *&out) {
     if (dc.relabel) { out = dc.relabel; return BCX_OK; }
     const HostComp &h = dc.host;
     const int32_t n = h.n;
     std::vector<int32_t> perm((size_t)n), inv((size_t)n);
-    for (int32_t v = 0; v < n; ++v) perm[v] = v;
-    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
-        return h.rp[a + 1] - h.rp[a] > h.rp[b + 1] - h.rp[b];
-    });
+    if (dc.rank.empty()) dc.rank = bfs_rank(h);
+    for (int32_t v = 0; v < n; ++v) perm[(size_t)dc.rank[v]] = v;
     for (int32_t i = 0; i < n; ++i) inv[perm[i]] = i;
     DComp *p = new DComp();
     p->n = n;

@@ -60,8 +60,9 @@ struct DComp {
     DBuf b_rp, b_col, b_reach, b_chp, b_chn, b_che;
     CompDev dev{};
     ChainDev chains{};
-    // multi-launch batched path: the component relabelled by descending
-    // degree (built on first use), perm[new] = old on the device, inv[old] = new
+    // multi-launch batched path: the component relabelled in BFS order from
+    // its highest-degree vertex (built on first use), perm[new] = old on the
+    // device, inv[old] = new
     DComp *relabel = nullptr;
     DBuf b_perm;
     std::vector<int32_t> relabel_inv;
