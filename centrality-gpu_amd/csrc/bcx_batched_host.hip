@@ -724,6 +724,11 @@ static int relabelled(bcx_ctx *ctx, DComp &dc, DComp *&out) {
     return BCX_OK;
 }
 
+int batched_prepare(bcx_ctx *ctx, DComp &dc) {
+    DComp *pc = nullptr;
+    return relabelled(ctx, dc, pc);
+}
+
 int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o, double *bc,
                       bcx_stats *st) {
     DComp *pc = nullptr;

@@ -151,6 +151,8 @@ int batched_persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg);
 void batched_persistent_reset(bcx_ctx *ctx);
 bool batched_narrow(bcx::DComp &dc);
 void small_release(bcx_ctx *ctx);
+// One-time preparation of a multi-launch component (its relabelled copy).
+int batched_prepare(bcx_ctx *ctx, bcx::DComp &dc);
 int batched_component(bcx_ctx *ctx, bcx::DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o,
                       double *bc, bcx_stats *st);
 int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o,

