@@ -584,6 +584,13 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
     bcx_stats *s = st ? st : &local;
     std::memset(s, 0, sizeof(*s));
     auto t0 = std::chrono::steady_clock::now();
+    // a run that failed part-way may have left work on the side stream and
+    // timer intervals open or uncollected
+    batched_persistent_reset(ctx);
+    ctx->timer.depth = 0;
+    ctx->timer.used = 0;
+    ctx->timer_chain.depth = 0;
+    ctx->timer_chain.used = 0;
     int rc = ctx->bc_acc.ensure(ctx, sizeof(double) * (size_t)std::max<int64_t>(g->total_n, 1), "bc accumulator");
     if (rc) return rc;
     double *bc = (double *)ctx->bc_acc.p;
@@ -658,8 +665,6 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
             (rc = ctx_hip(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "side stream")))
             return rc;
         const hipStream_t sp = overlap ? ctx->side : ctx->stream;
-        batched_persistent_reset(ctx);
-        ctx->timer.depth = 0;
         ctx->timer.begin(ctx);
         if (!small.empty() && (rc = batched_persistent_start(ctx, small, &o, bc, s, false, sp))) return rc;
         if (!wg.empty() && (rc = batched_persistent_start(ctx, wg, &o, bc, s, true, sp))) return rc;
