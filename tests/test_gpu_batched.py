@@ -99,12 +99,13 @@ def test_fp64_sigma_grid(engine):
 
 @pytest.mark.parametrize("prec", [bcx.PREC_FP64, bcx.PREC_FP32C])
 def test_sigma_word_escalation_through_hub_slices(engine, prec):
-    """A hub of degree k = 260 (its sigma summed by arc slices with atomics)
-    reaches sigma = k^4 = 4.6e9 > 2^32 from vertex 0 through four complete
-    bipartite layers: the tile leaves 32-bit words at the hub's level, its
-    forward stops there and it is rerun with 64-bit words.
+    """A hub of degree k = 520 (above the smallest hub threshold, 512 arcs,
+    so its sigma is summed by arc slices with atomics) reaches sigma = k^4 =
+    7.3e10 > 2^32 from vertex 0 through four complete bipartite layers: the
+    tile leaves 32-bit words at the hub's level, its forward stops there and
+    it is rerun with 64-bit words.
     Layers: 0 - L1 = L2 = L3 = L4 - hub (k vertices per layer)."""
-    k = 260
+    k = 520
     layers = [list(range(1 + i * k, 1 + (i + 1) * k)) for i in range(4)]
     hub = 1 + 4 * k
     edges = [(0, a) for a in layers[0]]
