@@ -148,7 +148,7 @@ typedef struct {
     int32_t        path;          /* BCX_PATH_*                              */
     int32_t        precision;     /* BCX_PREC_*                              */
     int32_t        use_chains;    /* 1: chain vertices via endpoint rows     */
-    int32_t        batch_width;   /* batched: concurrent tiles, 1..4 (0 = 4) */
+    int32_t        batch_width;   /* batched: concurrent tiles, 1..8 (0 = 8) */
     const int64_t *src_subset;    /* NULL = all sources; else slots          */
     int64_t        n_src;         /* slot = comp_offset[c] + local vertex    */
     int64_t        row_cap_bytes; /* endpoint-row slab cap (0 = auto)        */
