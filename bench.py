@@ -209,7 +209,7 @@ def main():
     c = comps[big]
     if not args.no_cpu:
         sys.path.insert(0, ROOT)
-        import oracle  # CPU baseline / checker only
+        import oracle  # CPU baseline and reference result only
         threads = oracle.num_threads()
         k = args.cpu_sources or 48 * threads
         rng = np.random.default_rng(1234)

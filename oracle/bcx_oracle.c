@@ -1,9 +1,9 @@
 /*
  * bcx_oracle.c - CPU restatement of the reference's exact-BC algorithm.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: the tests, the smoke
- * entry and the benchmark's CPU-baseline leg load it; the product path never
- * does.  Plain C99 + OpenMP, FP64 throughout.
+ * TEST INFRASTRUCTURE ONLY.  It computes the results the GPU path is compared
+ * against: the tests, the smoke entry and the benchmark's CPU-baseline leg load
+ * it; the product path never does.  Plain C99 + OpenMP, FP64 throughout.
  *
  * What each function restates (reference files, path relative to the
  * reference checkout):

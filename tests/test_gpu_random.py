@@ -4,8 +4,8 @@ Each graph (connected, simple, unit weight) is decomposed on the host and its
 whole-graph BC is computed by the plain path (FP64) and by the batched path
 with each kernel forced in turn: the small-component kernel, the
 workgroup-per-tile kernel and the multi-launch tiles, in FP64 and in
-FP32-compensated precision.  The checker is the oracle's FP64 Brandes on the
-whole graph.  Bars: FP64 1e-9, FP32C 1e-6 max relative error, zero BC exact.
+FP32-compensated precision, and compared with the oracle's FP64 Brandes on
+the whole graph.  Bars: FP64 1e-9, FP32C 1e-6 max relative error, zero BC exact.
 """
 import numpy as np
 import pytest
