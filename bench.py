@@ -11,8 +11,8 @@ run.  A step is credited n*m * (slice sources / job sources) traversals
 (MTEPS = n*m / t, n and m of the input graph).  Inputs are resident in HBM
 before the timed region; each step is one bcx_run through the C-ABI.
 
-One JSON line on stdout (rank 0, N=1).  --gpus N > 1 exits: this build runs on
-one GPU.
+One JSON line on stdout (rank 0, N=1).  --gpus N > 1 exits: the benchmark runs
+on one GPU.
 """
 import argparse
 import json
@@ -146,7 +146,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
     if not args.slices:

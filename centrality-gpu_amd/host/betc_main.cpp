@@ -5,7 +5,7 @@
 // Codes/kernels_free.cu:10-11,113, Codes/begin_gpupgm.cu:69,
 // Codes/kernels_active.cu:31,217,221, Codes/postprocesses.cu:200,
 // Codes/main.cpp:50,70) and the same side-effect files (PostProcessOp.txt,
-// Output.txt, parent.txt).  Timing values are this build's.
+// Output.txt, parent.txt).  Timing values are measured by this program.
 //
 // Extra, opt-in flags:
 //   --bc-out FILE        BC per vertex, "%d %f" (Codes/main.cpp:45-49)

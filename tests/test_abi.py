@@ -1,7 +1,9 @@
-"""The C-ABI library loads and exports every symbol include/bcx.h declares."""
+"""The C-ABI library loads and exports every symbol include/bcx.h declares;
+the CLI and the benchmark reject what they do not support."""
 import os
 import re
 import subprocess
+import sys
 
 import bcx
 
@@ -35,3 +37,10 @@ def test_cli_usage_exit_code():
     assert r.returncode == 1 and "Usage" in r.stdout
     r = subprocess.run([exe, "/nonexistent.txt"], capture_output=True, text=True)
     assert r.returncode == 2 and "cannot be opened" in r.stdout
+
+
+def test_bench_refuses_several_gpus():
+    root = os.path.dirname(bcx.HERE)
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and not r.stdout
