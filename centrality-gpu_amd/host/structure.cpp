@@ -350,8 +350,6 @@ static void block_cut(bcx_host *h, std::vector<std::vector<std::pair<int32_t,int
     reach_out.assign((size_t)nb, {});
     for (int32_t b = 0; b < nb; ++b)
         for (int32_t y : tadj[b]) {
-            int32_t v = -1;
-            (void)v;
             int64_t r = (par[y] == b) ? S[y] - 1 : (int64_t)n - S[b] - 1;
             reach_out[b].push_back({y - nb, (int32_t)r});
         }
