@@ -94,6 +94,9 @@ SIGNATURES = {
     "bcx_debug_source": (C.c_int, [C.c_void_p, C.POINTER(Component), C.c_int32, C.POINTER(Opts),
                                    _i32p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), _i32p]),
+    "bcx_debug_tile": (C.c_int, [C.c_void_p, C.POINTER(Component), _i32p, C.c_int32, C.POINTER(Opts),
+                                 _i32p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), _i32p, _i32p]),
     "bcx_generate": (C.c_int, [C.c_int32, C.c_uint64, C.c_double, _i32p, _i64p,
                                C.POINTER(_i32p)]),
     "bcx_batched_order": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _i32p]),
@@ -305,6 +308,7 @@ FLAG_NO_SMALL_KERNEL = 1
 FLAG_NO_GRAPH = 2
 FLAG_NO_WG_KERNEL = 4
 FLAG_WG_KERNEL = 8
+KERNEL_SMALL, KERNEL_WG, KERNEL_MULTI = 1, 2, 3
 
 
 def make_opts(path=PATH_PLAIN, precision=PREC_FP64, use_chains=True, subset=None,
@@ -396,6 +400,29 @@ class Engine:
         if rc:
             self._err(rc, "debug_source")
         return lv, su, sf, dl, bool(isf.value)
+
+    def debug_tile(self, comp, srcs, **kw):
+        """Every lane of one batched tile (bcx_debug_tile): levels [ns, n]
+        int32, sigma_u64 [ns, n], sigma_f64 [ns, n], delta [ns, n], whether
+        the counts are FP64, and the BCX_KERNEL_* that ran."""
+        kw.setdefault("path", PATH_BATCHED)
+        o, keep = make_opts(**kw)
+        n = comp.n
+        srcs = np.ascontiguousarray(np.asarray(srcs, np.int32))
+        ns = len(srcs)
+        lv = np.zeros((ns, n), np.int32)
+        su = np.zeros((ns, n), np.uint64)
+        sf = np.zeros((ns, n), np.float64)
+        dl = np.zeros((ns, n), np.float64)
+        isf = C.c_int32()
+        kern = C.c_int32()
+        cs = comp.struct()
+        rc = lib().bcx_debug_tile(self.h, C.byref(cs), _ptr(srcs), ns, C.byref(o), _ptr(lv),
+                                  _ptr(su, C.c_uint64), _ptr(sf, C.c_double), _ptr(dl, C.c_double),
+                                  C.byref(isf), C.byref(kern))
+        if rc:
+            self._err(rc, "debug_tile")
+        return lv, su, sf, dl, bool(isf.value), kern.value
 
 
 class DGraph:

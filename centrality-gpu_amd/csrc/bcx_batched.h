@@ -16,7 +16,7 @@ struct BatchDev {
     uint64_t *vis, *mb0, *mb1, *mb2;  // lanes reached; rotating level masks
     void *sig;                        // [n*64] u32, u64 or f64 bits (the tile's sigma mode)
     void *tt;                         // [n*64] float or double
-    uint16_t *lvl;                    // optional [n*64] levels (probe)
+    int32_t *lvl;                     // optional [n*64] levels (probe)
     double *dbg_delta;                // optional [n*64] dependencies (probe)
     int32_t *Q;                       // level lists, concatenated
     uint64_t *Qm;                     // lane mask of each list entry

@@ -302,7 +302,7 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
                     if (!F64 && (ovf || acc > sg_max<SM>() || (b.limit && acc >= b.limit))) b.ctr[1] = 1;
                 }
             }
-            if ((N & me) && b.lvl) b.lvl[(int64_t)w * 64 + lane] = (uint16_t)(l + 1);
+            if ((N & me) && b.lvl) b.lvl[(int64_t)w * 64 + lane] = l + 1;
             if (lane == 0) b.vis[w] |= N;
         }
         if (cnt >= STAGE_FLUSH) stage_flush(b, s_w[wv], s_m[wv], cnt, lane);

@@ -53,6 +53,10 @@ struct BatchedState {
     DBuf srcs, ws, hub_v, hub_nsl, sl_hub, sl_j0, sl_j1;
     int32_t src_cap = 0, n_hub = 0, n_slice = 0, hub_stride = 0, hub_deg = 0;
     bool probe = false;
+    // a failed run abandoned tiles part-way: level masks may hold bits of an
+    // unfinished forward and accumulators partial BC that was never folded;
+    // the next bs_ensure clears every slot before it is used again
+    bool dirty = false;
     int last_depth = 8;
     int last_sm = SIG_U32;  // sigma mode the last tile finished in: the next starts there
 };
@@ -66,7 +70,10 @@ static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
             hipHostMalloc((void **)&s.pinned, 64, hipHostMallocDefault) != hipSuccess)
             return ctx_fail(ctx, BCX_E_HIP, "batched slot stream");
     }
+    // capacities are recorded only once every buffer is in place: a failed
+    // ensure has released its buffer, so the next call must allocate again
     if (n > s.n_cap) {
+        s.n_cap = 0;
         if ((rc = s.vis.ensure(ctx, 8 * N, "batched vis")) || (rc = s.mb.ensure(ctx, 24 * N, "batched masks")) ||
             (rc = s.sig.ensure(ctx, 8 * 64 * N, "batched sigma")) ||
             (rc = s.tt.ensure(ctx, 8 * 64 * N, "batched dependency terms")) ||
@@ -80,13 +87,14 @@ static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
     }
     if (s.qcap < 8 * n + 4096) {
         const int64_t q = std::min<int64_t>((int64_t)8 * n + 4096, INT32_MAX / 2);
+        s.qcap = 0;
         if ((rc = s.Q.ensure(ctx, 4 * (size_t)q, "batched lists")) ||
             (rc = s.Qm.ensure(ctx, 8 * (size_t)q, "batched lists")))
             return rc;
         s.qcap = (int32_t)q;
     }
     if ((rc = s.ctr.ensure(ctx, 64, "batched counters"))) return rc;
-    if (probe && ((rc = s.lvl.ensure(ctx, 2 * 64 * N, "probe levels")) ||
+    if (probe && ((rc = s.lvl.ensure(ctx, 4 * 64 * N, "probe levels")) ||
                   (rc = s.dbg.ensure(ctx, 8 * 64 * N, "probe delta"))))
         return rc;
     return BCX_OK;
@@ -101,9 +109,22 @@ static int bs_ensure(bcx_ctx *ctx, int32_t n, int32_t n_src, int want_slots, boo
     int fit = (int)std::max<size_t>(1, std::min<size_t>(MAX_SLOTS, room / 2 / std::max<size_t>(per, 1)));
     bs.n_slots = std::max(1, std::min(want_slots, fit));
     int rc;
+    if (bs.dirty) {
+        // after the abandoned work on each slot stream
+        for (Slot &s : bs.slots) {
+            if (!s.stream || !s.n_cap) continue;
+            if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, s.stream), "memset")) ||
+                (rc = ctx_hip(ctx, hipMemsetAsync(s.bcacc.p, 0, s.bcacc.bytes, s.stream), "memset")) ||
+                (rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "slot reset")))
+                return rc;
+            s.tile = -1;
+        }
+        bs.dirty = false;
+    }
     for (int i = 0; i < bs.n_slots; ++i)
         if ((rc = slot_ensure(ctx, bs.slots[i], n, probe))) return rc;
     if (n_src > bs.src_cap) {
+        bs.src_cap = 0;
         if ((rc = bs.srcs.ensure(ctx, 4 * (size_t)n_src, "batched sources")) ||
             (rc = bs.ws.ensure(ctx, 8 * (size_t)n_src, "batched weights")))
             return rc;
@@ -126,7 +147,7 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uin
     b.mb2 = b.mb1 + s.n_cap;
     b.sig = s.sig.p;
     b.tt = s.tt.p;
-    b.lvl = bs.probe ? (uint16_t *)s.lvl.p : nullptr;
+    b.lvl = bs.probe ? (int32_t *)s.lvl.p : nullptr;
     b.dbg_delta = bs.probe ? (double *)s.dbg.p : nullptr;
     b.Q = (int32_t *)s.Q.p;
     b.Qm = (uint64_t *)s.Qm.p;
@@ -217,9 +238,13 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
                                                                                      INT32_MAX / 2));
         if (q2 <= s.qcap) { *rc_out = ctx_fail(ctx, BCX_E_NOMEM, "batched level lists cannot grow"); return 0; }
         if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, s.stream), "memset")) ||
-            (rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "sync")) ||
-            (rc = s.Q.ensure(ctx, 4 * (size_t)q2, "batched lists")) ||
-            (rc = s.Qm.ensure(ctx, 8 * (size_t)q2, "batched lists"))) {
+            (rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "sync"))) {
+            *rc_out = rc;
+            return 0;
+        }
+        s.qcap = 0;  // (a failed ensure has released the list it was growing)
+        if ((rc = s.Q.ensure(ctx, 4 * (size_t)q2, "batched lists (growth)")) ||
+            (rc = s.Qm.ensure(ctx, 8 * (size_t)q2, "batched lists (growth)"))) {
             *rc_out = rc;
             return 0;
         }
@@ -299,6 +324,7 @@ static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob
     ctx->timer.end(ctx);
     for (int i = 0; i < ns_slots; ++i) (void)hipStreamSynchronize(bs.slots[i].stream);
     (void)hipEventDestroy(e0);
+    if (rc) bs.dirty = true;
     if (sm_out) *sm_out = max_sm;
     return rc;
 }
@@ -769,7 +795,7 @@ int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs,
 // small components: one persistent launch over (component, 64-source) tiles
 // ------------------------------------------------------------------------
 struct SmallState {
-    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs;
+    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs, lvl, dbg;
     int grid = 0;
     // a pass started and not yet finished (persistent_start / _finish)
     bool pending = false, run_fp32 = false, run_graph = false;
@@ -846,7 +872,7 @@ static int persistent_pass(bcx_ctx *ctx, SmallState &ss, bool wg, bool f64, bool
 // Starts the pass on stream sp and returns without waiting for it;
 // persistent_finish waits, reruns the overflowed tiles and completes it.
 static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
-                            bcx_stats *st, bool wg, hipStream_t sp) {
+                            bcx_stats *st, bool wg, hipStream_t sp, bool probe) {
     if (jobs.empty()) return BCX_OK;
     if (!ctx->batched) ctx->batched = new BatchedState();
     SmallState *&state = wg ? ctx->wgk : ctx->small;
@@ -918,6 +944,9 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     if (wg && ((rc = ss.masks.ensure(ctx, 24 * (size_t)nmax * grid, "workgroup masks")) ||
                (rc = ss.qoffs.ensure(ctx, 4 * ((size_t)nmax + 2) * grid, "workgroup level offsets"))))
         return rc;
+    if (probe && ((rc = ss.lvl.ensure(ctx, 4 * slab * grid, "probe levels")) ||
+                  (rc = ss.dbg.ensure(ctx, 8 * slab * grid, "probe delta"))))
+        return rc;
     SmallArgs a{};
     a.comps = (const SmallComp *)ss.comps.p;
     a.tiles = (const SmallTile *)ss.tiles.p;
@@ -937,6 +966,8 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     a.qoffs = wg ? (int32_t *)ss.qoffs.p : nullptr;
     a.nmax = nmax;
     a.qcap = (int64_t)qcap;
+    a.lvl = probe ? (int32_t *)ss.lvl.p : nullptr;
+    a.dbg = probe ? (double *)ss.dbg.p : nullptr;
     ss.run = a;
     ss.run_fp32 = o->precision == BCX_PREC_FP32C;
     ss.run_graph = !wg && !(o->flags & BCX_FLAG_NO_GRAPH);
@@ -999,7 +1030,7 @@ static int persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg) {
 
 int batched_persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, const bcx_opts *o, double *bc_base,
                              bcx_stats *st, bool wg, hipStream_t sp) {
-    return persistent_start(ctx, jobs, o, bc_base, st, wg, sp);
+    return persistent_start(ctx, jobs, o, bc_base, st, wg, sp, false);
 }
 
 int batched_persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg) { return persistent_finish(ctx, st, wg); }
@@ -1015,7 +1046,7 @@ static void release_state(bcx_ctx *ctx, SmallState *&state) {
     SmallState &ss = *state;
     if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
     for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm,
-                    &ss.masks, &ss.qoffs})
+                    &ss.masks, &ss.qoffs, &ss.lvl, &ss.dbg})
         b->release(ctx);
     delete state;
     state = nullptr;
@@ -1026,56 +1057,109 @@ void small_release(bcx_ctx *ctx) {
     release_state(ctx, ctx->wgk);
 }
 
-int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o, int32_t *level,
-                         uint64_t *sigma_u64, double *sigma_f64, double *delta, int32_t *sigma_is_f64) {
+// Kernel the batched path runs component dc on, from the bcx_opts flags.
+int batched_kind(DComp &dc, int flags) {
+    if (dc.n <= SMALL_N && !(flags & BCX_FLAG_NO_SMALL_KERNEL)) return BCX_KERNEL_SMALL;
+    if (dc.n <= WG_N && !(flags & BCX_FLAG_NO_WG_KERNEL) && ((flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
+        return BCX_KERNEL_WG;
+    return BCX_KERNEL_MULTI;
+}
+
+// One tile of srcs[0..ns) on the kernel bcx_run would pick for the component
+// (the multi-launch tiles on the component's BFS-ordered copy, as in a run),
+// with the probe arrays of that kernel switched on; every lane is read back.
+int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns, const bcx_opts *o,
+                       int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta, int32_t *sigma_is_f64,
+                       int32_t *kernel_used) {
     bcx_dgraph *g = nullptr;
     int rc = bcx_upload(ctx, comp, 1, &g);
     if (rc) return rc;
     DComp &dc = g->comps[0];
     const int32_t n = dc.n;
-    rc = bs_ensure(ctx, n, BATCH_W, 1, true);
-    if (!rc) rc = ctx->bc_acc.ensure(ctx, sizeof(double) * n, "bc accumulator");
-    int sm = SIG_U32;
-    std::vector<uint16_t> lv((size_t)n * 64);
-    std::vector<uint64_t> sg((size_t)n * 64);
-    std::vector<double> dl((size_t)n * 64);
+    const int kind = batched_kind(dc, o->flags);
+    const size_t cells = (size_t)n * BATCH_W;
+    std::vector<int32_t> lv(cells, 0), at((size_t)n);  // at[v]: row of v in the device arrays
+    std::vector<uint64_t> sg(cells, 0);
+    std::vector<double> dl(cells, 0.0);
+    bool f64 = false;
+    batched_persistent_reset(ctx);
+    ctx->timer.depth = 0;
+    ctx->timer.used = 0;
+    double *bc = nullptr;
+    rc = ctx->bc_acc.ensure(ctx, sizeof(double) * n, "bc accumulator");
     if (!rc) {
-        std::vector<int32_t> tile{src};
-        rc = stage_sources(ctx, dc, tile);
-        if (!rc) rc = stage_hubs(ctx, dc);
-        if (!rc) rc = run_tiles(ctx, dc, 1, make_job(ctx, dc, o, (double *)ctx->bc_acc.p), nullptr, &sm);
+        bc = (double *)ctx->bc_acc.p;
+        rc = ctx_hip(ctx, hipMemsetAsync(bc, 0, sizeof(double) * n, ctx->stream), "memset");
     }
-    Slot &s = ctx->batched->slots[0];
-    // copies are ordered on the slot stream the kernels ran on
-    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), s.lvl.p, 2 * lv.size(), hipMemcpyDeviceToHost, s.stream),
-                          "copy");
-    const size_t wsz = sm == SIG_U32 ? 4 : 8;
-    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), s.sig.p, wsz * sg.size(), hipMemcpyDeviceToHost, s.stream),
-                          "copy");
-    if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), s.dbg.p, 8 * dl.size(), hipMemcpyDeviceToHost, s.stream),
-                          "copy");
-    if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "copy");
+    if (!rc && kind != BCX_KERNEL_MULTI) {
+        const bool wg = kind == BCX_KERNEL_WG;
+        for (int32_t v = 0; v < n; ++v) at[v] = v;
+        std::vector<SmallJob> jobs{{&dc, std::vector<int32_t>(srcs, srcs + ns)}};
+        bcx_stats st{};
+        rc = persistent_start(ctx, jobs, o, bc, &st, wg, ctx->stream, true);
+        if (!rc) rc = persistent_finish(ctx, &st, wg);
+        f64 = st.n_f64_sigma > 0;
+        // one tile, so one workgroup: its slab is the start of every array
+        const SmallState &ss = *(wg ? ctx->wgk : ctx->small);
+        if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), ss.lvl.p, 4 * cells, hipMemcpyDeviceToHost, ctx->stream),
+                              "copy");
+        if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), ss.sig.p, 8 * cells, hipMemcpyDeviceToHost, ctx->stream),
+                              "copy");
+        if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), ss.dbg.p, 8 * cells, hipMemcpyDeviceToHost, ctx->stream),
+                              "copy");
+        if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "copy");
+    } else if (!rc) {
+        DComp *pc = nullptr;
+        rc = relabelled(ctx, dc, pc);
+        int sm = SIG_U32;
+        if (!rc) {
+            std::vector<int32_t> ps((size_t)ns);
+            for (int32_t i = 0; i < ns; ++i) ps[(size_t)i] = dc.relabel_inv[srcs[i]];
+            for (int32_t v = 0; v < n; ++v) at[v] = dc.relabel_inv[v];
+            rc = bs_ensure(ctx, n, BATCH_W, 1, true);
+            if (!rc) rc = stage_sources(ctx, *pc, ps);
+            if (!rc) rc = stage_hubs(ctx, *pc);
+            if (!rc) {
+                TileJob j = make_job(ctx, *pc, o, bc);
+                j.perm = (const int32_t *)dc.b_perm.p;
+                rc = run_tiles(ctx, *pc, ns, j, nullptr, &sm);
+            }
+        }
+        if (ctx->batched) {
+            Slot &s = ctx->batched->slots[0];
+            const size_t wsz = sm == SIG_U32 ? 4 : 8;
+            // copies are ordered on the slot stream the kernels ran on
+            if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), s.lvl.p, 4 * cells, hipMemcpyDeviceToHost, s.stream),
+                                  "copy");
+            if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), s.sig.p, wsz * cells, hipMemcpyDeviceToHost, s.stream),
+                                  "copy");
+            if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), s.dbg.p, 8 * cells, hipMemcpyDeviceToHost, s.stream),
+                                  "copy");
+            if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "copy");
+            ctx->batched->probe = false;
+        }
+        f64 = sm == SIG_F64;
+        if (!rc && sm == SIG_U32) {  // 32-bit words: widen in place, from the top
+            const uint32_t *w32 = (const uint32_t *)sg.data();
+            for (size_t i = cells; i-- > 0;) sg[i] = w32[i];
+        }
+    }
     (void)ctx->timer.collect();
-    ctx->batched->probe = false;
     bcx_dgraph_free(ctx, g);
     if (rc) return rc;
-    const bool f64 = sm == SIG_F64;
-    if (sm == SIG_U32) {  // 32-bit words: widen lane 0 of every vertex
-        std::vector<uint64_t> w((size_t)n * 64, 0);
-        const uint32_t *sg32 = (const uint32_t *)sg.data();
-        for (size_t i = 0; i < w.size(); i += 64) w[i] = sg32[i];
-        sg.swap(w);
-    }
-    for (int32_t v = 0; v < n; ++v) {
-        const size_t i = (size_t)v * 64;
-        if (level) level[v] = lv[i];
-        double f;
-        std::memcpy(&f, &sg[i], 8);
-        if (sigma_u64) sigma_u64[v] = f64 ? 0 : sg[i];
-        if (sigma_f64) sigma_f64[v] = f64 ? f : (double)sg[i];
-        if (delta) delta[v] = v == src ? 0.0 : dl[i];
-    }
+    for (int32_t i = 0; i < ns; ++i)
+        for (int32_t v = 0; v < n; ++v) {
+            const size_t c = (size_t)at[v] * BATCH_W + (size_t)i, r = (size_t)i * (size_t)n + (size_t)v;
+            double f;
+            std::memcpy(&f, &sg[c], 8);
+            if (level) level[r] = lv[c];
+            if (sigma_u64) sigma_u64[r] = f64 ? 0 : sg[c];
+            if (sigma_f64) sigma_f64[r] = f64 ? f : (double)sg[c];
+            // a lane's own source has no dependency (it is never a backward entry)
+            if (delta) delta[r] = v == srcs[i] ? 0.0 : dl[c];
+        }
     if (sigma_is_f64) *sigma_is_f64 = f64 ? 1 : 0;
+    if (kernel_used) *kernel_used = kind;
     return BCX_OK;
 }
 

@@ -155,8 +155,11 @@ void small_release(bcx_ctx *ctx);
 int batched_prepare(bcx_ctx *ctx, bcx::DComp &dc);
 int batched_component(bcx_ctx *ctx, bcx::DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o,
                       double *bc, bcx_stats *st);
-int batched_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const bcx_opts *o,
-                         int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta,
-                         int32_t *sigma_is_f64);
+// Kernel (BCX_KERNEL_*) the batched path runs a component on, given the
+// bcx_opts flags.
+int batched_kind(bcx::DComp &dc, int flags);
+int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns, const bcx_opts *o,
+                       int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta, int32_t *sigma_is_f64,
+                       int32_t *kernel_used);
 void batched_release(bcx_ctx *ctx);
 void batched_graph_release(bcx_ctx *ctx, bcx_dgraph *g);

@@ -16,6 +16,7 @@
 
 #include "../host/host_graph.h"
 #include "bcx.h"
+#include "bcx_batched.h"
 #include "bcx_device.h"
 #include "bcx_kernels.h"
 #include "bcx_runtime.h"
@@ -641,15 +642,15 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         int64_t n_chain = 0;
         std::vector<int32_t> srcs = batched_sources(dc, sp, sn, o.use_chains ? &n_chain : nullptr);
         account((int64_t)srcs.size() - n_chain, n_chain);
-        if (dc.n <= SMALL_N && !(o.flags & BCX_FLAG_NO_SMALL_KERNEL)) {
+        const int kind = batched_kind(dc, o.flags);
+        if (kind == BCX_KERNEL_SMALL) {
             small.push_back({&dc, std::move(srcs)});
             continue;
         }
         // larger components: tile order from the tuner (BFS ranks or balls)
         std::vector<int32_t> tiles = tuned_sources(dc, srcs);
         if (dc.tile_order == 1) s->n_ball_tiles++;
-        if (dc.n <= WG_N && !(o.flags & BCX_FLAG_NO_WG_KERNEL) &&
-            ((o.flags & BCX_FLAG_WG_KERNEL) || batched_narrow(dc)))
+        if (kind == BCX_KERNEL_WG)
             wg.push_back({&dc, std::move(tiles)});
         else {
             if ((rc = batched_prepare(ctx, dc))) return rc;  // one-time work stays outside the timing
@@ -733,8 +734,8 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
     if (!ctx || !comp || src < 0 || src >= comp->n) return ctx_fail(ctx, BCX_E_ARG, "bad debug args");
     bcx_opts o{};
     if (opts) o = *opts;
-    if (o.path == BCX_PATH_BATCHED) return batched_debug_source(ctx, comp, src, &o, level, sigma_u64,
-                                                                sigma_f64, delta, sigma_is_f64);
+    if (o.path == BCX_PATH_BATCHED)
+        return batched_debug_tile(ctx, comp, &src, 1, &o, level, sigma_u64, sigma_f64, delta, sigma_is_f64, nullptr);
     bcx_dgraph *g = nullptr;
     int rc = bcx_upload(ctx, comp, 1, &g);
     if (rc) return rc;
@@ -804,6 +805,24 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
     if (delta) std::memcpy(delta, dl.data(), 8 * n);
     if (sigma_is_f64) *sigma_is_f64 = f64 ? 1 : 0;
     return BCX_OK;
+}
+
+int bcx_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns, const bcx_opts *opts,
+                   int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta, int32_t *sigma_is_f64,
+                   int32_t *kernel_used) {
+    if (!ctx || !comp || !srcs || ns < 1 || ns > BATCH_W || comp->n <= 0)
+        return ctx_fail(ctx, BCX_E_ARG, "bad debug tile args");
+    std::vector<uint8_t> seen((size_t)comp->n, 0);
+    for (int32_t i = 0; i < ns; ++i) {
+        if (srcs[i] < 0 || srcs[i] >= comp->n || seen[(size_t)srcs[i]])
+            return ctx_fail(ctx, BCX_E_ARG, "tile sources must be distinct vertices of the component");
+        seen[(size_t)srcs[i]] = 1;
+    }
+    (void)hipSetDevice(ctx->device);
+    bcx_opts o{};
+    if (opts) o = *opts;
+    return batched_debug_tile(ctx, comp, srcs, ns, &o, level, sigma_u64, sigma_f64, delta, sigma_is_f64,
+                              kernel_used);
 }
 
 }  // extern "C"

@@ -43,6 +43,10 @@ struct SmallArgs {
     int32_t *qoffs;     // [grid][nmax + 2]
     int64_t nmax;       // largest component
     int64_t qcap;       // list entries per workgroup (64 * nmax + 64)
+    // probe only (null in runs): per-workgroup slabs like sig, holding the
+    // level and the dependency of every (vertex, lane) of the last tile
+    int32_t *lvl;
+    double *dbg;
 };
 
 hipError_t launch_small(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st);

@@ -55,6 +55,8 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
     TT *tt = (TT *)a.tt + (int64_t)blockIdx.x * a.slab;
     int32_t *Q = a.Q + (int64_t)blockIdx.x * a.slab;
     uint64_t *Qm = a.Qm + (int64_t)blockIdx.x * a.slab;
+    int32_t *plv = a.lvl ? a.lvl + (int64_t)blockIdx.x * a.slab : nullptr;
+    double *pdl = a.dbg ? a.dbg + (int64_t)blockIdx.x * a.slab : nullptr;
     while (true) {
         if (tid == 0) s_item = atomicAdd(&a.counter[0], 1);
         __syncthreads();
@@ -83,6 +85,7 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
             sig[(int64_t)s * 64 + tid] = F64 ? d2u(1.0) : 1ull;
             Q[tid] = s;
             Qm[tid] = 1ull << tid;
+            if (plv) plv[(int64_t)s * 64 + tid] = 0;
         }
         __syncthreads();
         uint64_t *cur = mA, *nxt = mB;
@@ -193,6 +196,7 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
                 if (N & me) {
                     sig[(int64_t)w * 64 + lane] = F64 ? d2u(facc) : acc;
                     if (!F64 && (ovf || (a.limit && acc >= a.limit))) s_ovf = 1;
+                    if (plv) plv[(int64_t)w * 64 + lane] = l + 1;
                 }
                 if (lane == 0) vis[w] |= N;
             }
@@ -269,6 +273,7 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
                     const double delta = sv * s;
                     contrib = a.ws[tl.src0 + lane] * delta;
                     tt[(int64_t)v * 64 + lane] = (TT)(((double)cp.reach[v] + 1.0) / sv + s);
+                    if (pdl) pdl[(int64_t)v * 64 + lane] = delta;
                 }
                 contrib = wave_sum(contrib);
                 if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&cp.bc[v], contrib);

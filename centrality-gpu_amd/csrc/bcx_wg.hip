@@ -148,7 +148,7 @@ struct EntryMeta {
 template <bool F64>
 __device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *col, const uint64_t *cur, uint64_t *Qm,
                                            uint64_t *sig, uint64_t *vis, int e0, int cnt, EntryMeta em, WaveLds &L,
-                                           int *s_ovf) {
+                                           int *s_ovf, int32_t *plv, int level) {
     const int lane = threadIdx.x & 63;
     const uint64_t me = 1ull << lane;
     const int w0 = em.v, r0 = em.r0, r1 = em.r1;
@@ -166,6 +166,7 @@ __device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *col, c
         if (N & me) {
             sig[(int64_t)w * 64 + lane] = val;
             if (!F64 && limit && val >= limit) ovf = true;
+            if (plv) plv[(int64_t)w * 64 + lane] = level;
         }
         if (lane == 0) atomicOr((unsigned long long *)&vis[w], (unsigned long long)N);  // result unused: no return
         acc = 0;
@@ -222,7 +223,7 @@ __device__ __forceinline__ void pull_block(uint64_t limit, const int32_t *col, c
 template <bool F64, typename TT>
 __device__ __forceinline__ void back_block(const int32_t *col, const int32_t *reach, double *bc, const double *ws,
                                            const uint64_t *succ, uint64_t *own, const uint64_t *sig, TT *tt, int cnt,
-                                           bool has_succ, EntryMeta em, WaveLds &L) {
+                                           bool has_succ, EntryMeta em, WaveLds &L, double *pdl) {
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanW, SumW>::type;
     constexpr int BUNR = sizeof(TT) == 8 ? GUNR : 2 * GUNR;  // 32-bit rows: twice as many in flight
     const int lane = threadIdx.x & 63;
@@ -284,6 +285,7 @@ __device__ __forceinline__ void back_block(const int32_t *col, const int32_t *re
             const double delta = sv * rho;
             contrib = ws[lane] * delta;
             tt[(int64_t)v * 64 + lane] = (TT)(((double)reach[v] + 1.0) / sv + rho);
+            if (pdl) pdl[(int64_t)v * 64 + lane] = delta;
         }
         contrib = wave_sum(contrib);
         if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&bc[v], contrib);
@@ -303,6 +305,8 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
     uint64_t *vis = a.masks + (int64_t)blockIdx.x * 3 * nm;
     uint64_t *mA = vis + nm, *mB = mA + nm;
     int32_t *qoff = a.qoffs + (int64_t)blockIdx.x * (nm + 2);
+    int32_t *plv = a.lvl ? a.lvl + (int64_t)blockIdx.x * a.slab : nullptr;
+    double *pdl = a.dbg ? a.dbg + (int64_t)blockIdx.x * a.slab : nullptr;
     while (true) {
         if (tid == 0) s_item = atomicAdd(&a.counter[0], 1);
         wg_barrier();
@@ -331,6 +335,7 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
             sig[(int64_t)s * 64 + tid] = F64 ? d2u(1.0) : 1ull;
             Q[tid] = s;
             Qm[tid] = 1ull << tid;
+            if (plv) plv[(int64_t)s * 64 + tid] = 0;
         }
         wg_barrier();
         uint64_t *cur = mA, *nxt = mB;
@@ -421,7 +426,8 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
                         em = meta(e0 + S, qn);
                         qn = (e0 + 2 * S + lane < q2 && lane < GEB) ? Q[e0 + 2 * S + lane] : 0;
                     }
-                    pull_block<F64>(a.limit, col, cur, Qm, sig, vis, e0, min(GEB, q2 - e0), cur_em, lds[wv], &s_ovf);
+                    pull_block<F64>(a.limit, col, cur, Qm, sig, vis, e0, min(GEB, q2 - e0), cur_em, lds[wv], &s_ovf,
+                                    plv, l + 1);
                 }
             }
             wg_barrier();
@@ -485,7 +491,7 @@ __global__ __launch_bounds__(GB) void k_wg(SmallArgs a) {
                         entry(e0 + 2 * S, en);
                     }
                     back_block<F64, TT>(col, cp.reach, cp.bc, a.ws + tl.src0, succ, own, sig, tt, min(BEB, b1 - e0),
-                                        lb < D, cur_em, lds[wv]);
+                                        lb < D, cur_em, lds[wv], pdl);
                 }
             }
             wg_barrier();

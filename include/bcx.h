@@ -237,6 +237,27 @@ int  bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src,
                       const bcx_opts *opts, int32_t *level, uint64_t *sigma_u64,
                       double *sigma_f64, double *delta, int32_t *sigma_is_f64);
 
+/* Kernels of the batched path (bcx_debug_tile's *kernel_used).            */
+#define BCX_KERNEL_SMALL     1   /* workgroup per tile, state in LDS (n <= 2048)  */
+#define BCX_KERNEL_WG        2   /* workgroup per tile, state in HBM (deep BFS)   */
+#define BCX_KERNEL_MULTI     3   /* level launches across the device per tile     */
+
+/* Parity probe of one whole tile of the batched path: srcs[0..ns) (1 <= ns
+ * <= 64, distinct local vertices) run as the lanes of one tile on the kernel
+ * that bcx_run picks for this component under opts->flags (the flags force
+ * a kernel as they do for a run), with that kernel's per-lane probe arrays
+ * switched on; every lane is read back.  Outputs are [ns * n], row i for
+ * srcs[i] (any pointer may be NULL): levels, path counts (both forms; the
+ * tile's counts are FP64 when *sigma_is_f64 = 1, else exact integers) and
+ * dependencies.  *kernel_used receives the BCX_KERNEL_* that ran.  Extends
+ * bcx_debug_source (one source; the reference's per-source rows are
+ * Codes/kernels_active.cu:544-552) to the 64 lanes of a tile, which the
+ * reference, one source per block, does not have.                         */
+int  bcx_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns,
+                    const bcx_opts *opts, int32_t *level, uint64_t *sigma_u64,
+                    double *sigma_f64, double *delta, int32_t *sigma_is_f64,
+                    int32_t *kernel_used);
+
 /* ======================================================================= */
 /* 3. Synthetic workloads                                                    */
 /* ======================================================================= */

@@ -27,6 +27,9 @@ def lib():
         L.orc_source.restype = C.c_int
         L.orc_source.argtypes = [C.c_int32, _i32p, _i32p, _i32p, C.c_int32, _i32p,
                                  C.POINTER(C.c_uint64), dp, dp]
+        L.orc_sources.restype = None
+        L.orc_sources.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _i32p, C.c_int32, _i32p,
+                                  C.POINTER(C.c_uint64), dp, dp, _i32p, C.c_int32]
         L.orc_chain_source.restype = C.c_int
         L.orc_chain_source.argtypes = [C.c_int32, _i32p, _i32p, _i32p, C.c_int32, C.c_int32,
                                        C.c_int32, _i32p, C.c_int32, _i32p,
@@ -79,6 +82,25 @@ def source(rp, col, reach, s):
     ovf = lib().orc_source(n, _p(rp), _p(col), _p(reach), s, _p(lv), _p(su, C.c_uint64),
                            _p(sf, C.c_double), _p(dl, C.c_double))
     return lv, su, sf, dl, bool(ovf)
+
+
+def sources(rp, col, reach, srcs, threads=0):
+    """source() for every s in srcs on host threads: levels [ns, n] int32,
+    sigma_u64 [ns, n], sigma_f64 [ns, n], delta [ns, n], overflow [ns] bool."""
+    n = len(rp) - 1
+    rp = np.ascontiguousarray(rp, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    reach = np.zeros(n, np.int32) if reach is None else np.ascontiguousarray(reach, np.int32)
+    srcs = np.ascontiguousarray(srcs, np.int32)
+    ns = len(srcs)
+    lv = np.zeros((ns, n), np.int32)
+    su = np.zeros((ns, n), np.uint64)
+    sf = np.zeros((ns, n), np.float64)
+    dl = np.zeros((ns, n), np.float64)
+    ovf = np.zeros(ns, np.int32)
+    lib().orc_sources(n, _p(rp), _p(col), _p(reach), _p(srcs), ns, _p(lv), _p(su, C.c_uint64),
+                      _p(sf, C.c_double), _p(dl, C.c_double), _p(ovf), threads)
+    return lv, su, sf, dl, ovf.astype(bool)
 
 
 def chain_source(rp, col, reach, L, R, nodes, pos):

@@ -15,6 +15,8 @@
  *                       dependency of the successor method
  *                       delta_w = sum_v sigma_w/sigma_v ((r(v)+1) + delta_v)
  *                       (Codes/kernels_free.cu:378-421).
+ *   orc_sources         orc_source for a list of sources on host threads
+ *                       (one call per source, so pinned as orc_source is).
  *   orc_chain_source    the chain-source combine of
  *                       Codes/postprocesses.cu:329-360 (min over the two
  *                       endpoint rows plus ld/rd, sigma summed on ties, then
@@ -142,6 +144,24 @@ int orc_source(int32_t n, const int32_t *rp, const int32_t *col, const int32_t *
     if (delta) memcpy(delta, dl, sizeof(double) * n);
     free(lvl); free(order); free(su); free(sf); free(sig); free(dl);
     return ovf;
+}
+
+/* orc_source for srcs[0..ns), OpenMP over sources: row i of each output
+ * ([ns * n], any may be NULL) and ovf[i] belong to srcs[i].               */
+void orc_sources(int32_t n, const int32_t *rp, const int32_t *col, const int32_t *reach,
+                 const int32_t *srcs, int32_t ns, int32_t *level, uint64_t *sigma_u64,
+                 double *sigma_f64, double *delta, int32_t *ovf, int32_t threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < ns; ++i) {
+        const size_t o = (size_t)i * (size_t)n;
+        int r = orc_source(n, rp, col, reach, srcs[i], level ? level + o : NULL,
+                           sigma_u64 ? sigma_u64 + o : NULL, sigma_f64 ? sigma_f64 + o : NULL,
+                           delta ? delta + o : NULL);
+        if (ovf) ovf[i] = r;
+    }
 }
 
 /* Chain source x = chain_nodes[pos] of a chain with endpoints L,R and cnt

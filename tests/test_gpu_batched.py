@@ -117,10 +117,10 @@ def test_sigma_word_escalation_through_hub_slices(engine, prec):
     gid = list(c.gid)
     _, su, _, _, _ = oracle.source(c.row_ptr, c.col, c.reach, gid.index(0))
     assert int(su[gid.index(hub)]) == k ** 4 > 2 ** 32
-    for s in (0, hub, 5, 3 * k):
-        check_source(engine, c, gid.index(s), prec)
-    want = oracle.component_bc(c.row_ptr, c.col, c.reach)
     flags = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_NO_WG_KERNEL
+    for s in (0, hub, 5, 3 * k):
+        check_source(engine, c, gid.index(s), prec, flags=flags)
+    want = oracle.component_bc(c.row_ptr, c.col, c.reach)
     got, _ = engine.bc([c], path=B, precision=prec, flags=flags)
     mx, _, zeros = relerr(got[0], want)
     assert mx < (1e-6 if prec == bcx.PREC_FP32C else 1e-9) and zeros
@@ -318,3 +318,58 @@ def test_wg_kernel_threshold_and_subset(engine):
         assert mx < 1e-9 and zeros
     finally:
         g.free()
+
+
+def test_failed_run_leaves_context_clean():
+    """A multi-launch run that fails part-way -- the device-memory cap is
+    reached while a tile's level lists grow (ball tiles of a 60x60 grid hold
+    ~14 list entries per vertex, the lists start at 8) -- abandons the tiles
+    in flight on the other slots.  The next run on the same context, on a
+    smaller component, must still equal the oracle: no level-mask bits of the
+    abandoned forwards and no unfolded partial BC may survive."""
+    flags = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_NO_WG_KERNEL
+    grid = comps_of(3600, grid_edges(60, 60))[0]
+    n, uv = bcx.generate(2, 0, 0.003)
+    core = max(comps_of(n, uv), key=lambda c: c.n)
+    assert 2048 < core.n < grid.n
+    want = oracle.component_bc(core.row_ptr, core.col, core.reach)
+
+    def attempt(cap):
+        try:
+            eng = bcx.Engine(0, mem_cap=cap)
+        except bcx.BcxError:
+            return None, None
+        try:
+            eng.bc([grid], path=B, precision=bcx.PREC_FP64, flags=flags)
+            return eng, ""
+        except bcx.BcxError as e:
+            return eng, str(e)
+
+    lo, hi = 1 << 20, 1 << 30  # the grid fails at lo and runs at hi
+    while hi - lo > 4096:
+        mid = (lo + hi) // 2
+        eng, msg = attempt(mid)
+        if eng is not None:
+            eng.close()
+        if msg == "":
+            hi = mid
+        else:
+            lo = mid
+    print("smallest cap that runs the grid:", hi, flush=True)
+    for k in range(1, 33):
+        eng, msg = attempt(hi - 4096 * k)
+        if eng is None:
+            continue
+        if "growth" not in msg:
+            eng.close()
+            continue
+        try:
+            print("failed as wanted:", msg, flush=True)
+            for prec, tol in ((bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)):
+                got, st = eng.bc([core], path=B, precision=prec, flags=flags)
+                mx, _, zeros = relerr(got[0], want)
+                assert mx < tol and zeros, (prec, mx)
+        finally:
+            eng.close()
+        return
+    pytest.fail("no memory cap made the list growth fail")

@@ -115,6 +115,25 @@ def test_sigma_overflow_flag():
     assert not ovf
 
 
+def test_sources_rows_equal_single_sources():
+    """oracle.sources (threaded, many sources) gives row by row exactly what
+    oracle.source gives, including the overflow flag and the FP64 shadow;
+    a grid corner (overflow) and an interior vertex (no overflow) included."""
+    w = h = 40
+    rp, col = csr(w * h, grid_edges(w, h))
+    reach = np.arange(w * h, dtype=np.int32) % 3
+    srcs = np.array([0, 820, 5, 1599, 41], np.int32)
+    lv, su, sf, dl, ovf = oracle.sources(rp, col, reach, srcs, threads=3)
+    for i, s in enumerate(srcs):
+        a = oracle.source(rp, col, reach, int(s))
+        np.testing.assert_array_equal(lv[i], a[0])
+        np.testing.assert_array_equal(su[i], a[1])
+        np.testing.assert_array_equal(sf[i].view(np.uint64), a[2].view(np.uint64))
+        np.testing.assert_array_equal(dl[i].view(np.uint64), a[3].view(np.uint64))
+        assert ovf[i] == a[4]
+    assert ovf[0] and not ovf[1]
+
+
 def test_sigma_complete_and_cycle():
     rp, col = csr(6, [(i, j) for i in range(6) for j in range(i + 1, 6)])
     lv, su, *_ = oracle.source(rp, col, None, 2)
