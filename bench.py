@@ -86,6 +86,48 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "copies": 3}))
 
 
+BATCHED_KERNELS = ("bcx::k_b_", "bcx::k_small", "bcx::k_wg", "bcx::k_na_")
+
+
+def _union(iv):
+    """Total length of the union of [start, end) intervals."""
+    tot, cur_s, cur_e = 0, None, None
+    for s, e in sorted(iv):
+        if cur_e is None or s > cur_e:
+            if cur_e is not None:
+                tot += cur_e - cur_s
+            cur_s, cur_e = s, e
+        else:
+            cur_e = max(cur_e, e)
+    if cur_e is not None:
+        tot += cur_e - cur_s
+    return tot
+
+
+def summarize_trace(path):
+    """Markdown summary of a rocprofv3 kernel-trace CSV: per kernel the calls,
+    the summed duration and the busy time as the union of its launches'
+    intervals (kernels of concurrent tiles overlap, so sums exceed the wall
+    time); for the batched kernels together, their union and the span from the
+    first start to the last end."""
+    import csv
+    rows = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            rows.setdefault(r["Kernel_Name"], []).append((s, e))
+    print("| Kernel | Calls | Sum ms | Union ms | Avg us |")
+    print("|---|---:|---:|---:|---:|")
+    tot = lambda iv: sum(e - s for s, e in iv)  # noqa: E731
+    for k, iv in sorted(rows.items(), key=lambda kv: -tot(kv[1])):
+        print(f"| `{k[:60]}` | {len(iv)} | {tot(iv) / 1e6:.1f} | {_union(iv) / 1e6:.1f} | "
+              f"{tot(iv) / len(iv) / 1e3:.1f} |")
+    b = [x for k, iv in rows.items() if k.startswith(BATCHED_KERNELS) for x in iv]
+    if b:
+        print(f"\nBatched kernels: {len(b)} launches, union {_union(b) / 1e6:.1f} ms, "
+              f"span first start to last end {(max(e for _, e in b) - min(s for s, _ in b)) / 1e6:.1f} ms")
+
+
 TRAFFIC_FILES = {2: "batched_config2_traffic.json", 4: "wg_config4_traffic.json"}
 WORKLOADS = {
     1: "16 Erdos-Renyi blocks of 128 vertices on a tree of cut vertices, 20% of edges as chains, "
@@ -134,16 +176,22 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--slices", type=int, default=0,
                     help="0 = per workload: 64 for the wide graphs, 1 for the deep grid (4)")
-    ap.add_argument("--plain-slices", type=int, default=1024)
-    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-sources", type=int, default=4096,
+                    help="sources the plain path is timed on (whole tiles, evenly spaced over the job; "
+                         "8x the plain path's persistent grid of 512 workgroups)")
     ap.add_argument("--cpu-sources", type=int, default=0, help="0 = 48 per host thread (10-30 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only a device copy (profiling)")
+    ap.add_argument("--summarize-trace", default="",
+                    help="rocprofv3 kernel-trace CSV: print per-kernel calls, summed and union busy time")
     ap.add_argument("--slots", type=int, default=0, help="concurrent tiles of the batched path (0 = library default)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
         return
     if args.gpus != 1:
         print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -174,7 +222,9 @@ def main():
     t0 = time.perf_counter()
     orders = [g.batched_order(ci) for ci in range(len(comps))]
     slices = make_slices(comps, offsets, orders, args.slices)
-    pslices = make_slices(comps, offsets, orders, args.plain_slices)
+    n_tiles = sum((c.n + 63) // 64 for c in comps)
+    plain_slices = max(1, n_tiles // max(1, args.plain_sources // 64))
+    pslices = make_slices(comps, offsets, orders, plain_slices)
     print(f"# tile orders {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
     W = float(n) * float(m)
 
@@ -192,15 +242,24 @@ def main():
 
     plain = None
     if not args.no_plain:
-        pidx = [i % args.plain_slices for i in range(args.plain_steps)]
-        t_p, tk_p, st_p = timed_steps(g, pslices, pidx, 0, path=bcx.PATH_PLAIN,
-                                      precision=bcx.PREC_FP64, use_chains=True)
-        src_p = sum(len(pslices[i][0]) for i in pidx)
-        mteps_p = W * src_p / total_src / t_p / 1e6
-        balg_p = sum(bytes_alg(comps, pslices[i][1], pslices[i][2]) for i in pidx)
-        plain = {"mteps": mteps_p, "s": t_p, "kernel_s": tk_p, "sources": src_p,
-                 "achieved_gbs": balg_p / tk_p / 1e9 if tk_p > 0 else None}
-        print(f"# plain: {len(pidx)} steps {t_p:.3f}s {mteps_p:.1f} MTEPS", file=sys.stderr, flush=True)
+        # one slice of whole tiles (stride plain_slices over the tile order):
+        # with chain reduction (the reference's method: chain vertices combine
+        # two endpoint rows, which may need forward-only row passes), and
+        # without (every vertex a BFS source); the faster one is the comparison
+        sub = pslices[0][0]
+        g.run(subset=sub[:64], path=bcx.PATH_PLAIN, precision=bcx.PREC_FP64)  # module load, plan buffers
+        plain = {"sources": int(len(sub)), "tiles_in_sample": int((len(sub) + 63) // 64)}
+        for key, chains in (("chains", True), ("no_chains", False)):
+            t0 = time.perf_counter()
+            _, st = g.run(subset=sub, path=bcx.PATH_PLAIN, precision=bcx.PREC_FP64, use_chains=chains)
+            t_p = time.perf_counter() - t0
+            plain[key] = {"mteps": W * len(sub) / total_src / t_p / 1e6, "s": t_p, "kernel_s": st["t_kernel_s"],
+                          "bfs_sources": st["n_bfs_sources"], "chain_sources": st["n_chain_sources"],
+                          "row_only_passes": st["n_row_only"],
+                          "achieved_gbs": st["bytes_alg"] / st["t_kernel_s"] / 1e9 if st["t_kernel_s"] else None}
+            print(f"# plain ({key}): {len(sub)} sources {t_p:.3f}s {plain[key]['mteps']:.1f} MTEPS "
+                  f"(row-only passes {st['n_row_only']})", file=sys.stderr, flush=True)
+        plain["mteps"] = max(plain["chains"]["mteps"], plain["no_chains"]["mteps"])
 
     # precision + CPU baseline on one bounded sample of the giant component
     cpu = None
@@ -237,6 +296,7 @@ def main():
 
     achieved = balg_b / tk_b / 1e9 if tk_b > 0 else None
     traffic = measured_traffic(args.config, src_b / K) if args.scale == 1.0 else None
+    launches = sum(st["n_launches"] for st in st_b) / K
     line = {
         "metric": "exact-BC MTEPS on 1 MI355X: batched FP32-comp vs plain, <=1e-6 rel err",
         "value": round(mteps_b, 3),
@@ -261,6 +321,10 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": round(traffic) if traffic else None,
                      "traffic_unit": "HBM bytes per step (rocprofv3 FETCH_SIZE+WRITE_SIZE, scaled)",
+                     # the same bytes over the same time: the memory system's actual rate
+                     "frac_measured_bytes": round(traffic / (tk_b / K) / 1e9 / HBM_PEAK_GBS, 4)
+                     if traffic and tk_b > 0 else None,
+                     "launches_per_step": round(launches, 1),
                      "algorithmic_bytes_per_step": round(balg_b / K),
                      "kernel": "batched kernels of a step (k_b_expand, k_b_pull, k_b_back; k_small; k_wg)"},
         "cpu_baseline": cpu,

@@ -203,7 +203,13 @@ int main(int argc, char **argv) {
         }
         if (comp.n_chains > 0) {
             std::printf("max_threads_per_block %d number_of_SMs %d\n", comp.n < 1024 ? comp.n : 1024, n_cu);
-            std::printf("postprocess time is %f\n", t_post);
+            // the reference postprocesses every endpoint batch after the first
+            // (Codes/kernels_active.cu:98-158), batches holding fewer than
+            // 2000 endpoints (Codes/arrangelevels.cpp:8,84-109): with fewer
+            // endpoints there is one batch and no such line.  (With more, the
+            // reference's batch count follows its ear-graph BFS levels; one
+            // line is printed here.)
+            if (x.ne >= 2000) std::printf("postprocess time is %f\n", t_post);
             std::printf("total time for active vertices is %f\n", t_act);
             std::printf("total_act_vert_size is %d\n", x.ne);
             active_total += x.ne;

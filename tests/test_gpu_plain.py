@@ -183,6 +183,65 @@ def test_cli_sample_inputs(goldens, tmp_path):
         assert (tmp_path / "PostProcessOp.txt").exists()
 
 
+def _expected_stdout(ref):
+    """The reference's stdout for an input, numbers that are timings or
+    machine properties blanked as 'T': structure lines from the independent
+    restatement (tests/golden/make_goldens.py) and, per block of more than
+    two vertices, the lines its run prints (Codes/bcc_bfs.cpp:310;
+    Codes/kernels_free.cu:10-11,113 and Codes/begin_gpupgm.cu:69 when the
+    block has free vertices; Codes/kernels_active.cu:31,217,221 when it has
+    chains -- with fewer than 2000 endpoints there is one endpoint batch and
+    no postprocess line, Codes/kernels_active.cu:98-158)."""
+    lines = [ref["lines"][0]]
+    blocks = {b[0]: b for b in ref["blocks"]}
+    for l in ref["lines"][1:-1]:
+        i = int(l.split()[3])
+        _, nf, ne = blocks[i]
+        n = int(l.split()[5])
+        lines += [l, "Num max threads T"]
+        if nf > 0:
+            lines += ["Inside bc_gpu_free ", f" Free vertices count is {nf}",
+                      "total time for free vertices is ==================== T", "Free vertices processing finished"]
+        if ne > 0:
+            assert ne < 2000
+            lines += [f"max_threads_per_block {min(n, 1024)} number_of_SMs T",
+                      "total time for active vertices is T", f"total_act_vert_size is {ne}"]
+    return lines + [ref["lines"][-1], "GPU total time is T "]
+
+
+def _blank(stdout):
+    import re
+    out = []
+    for l in stdout.splitlines():
+        if l.startswith("Num max threads ") or l.startswith("max_threads_per_block "):
+            l = re.sub(r"\d+$", "T", l)
+        elif "time" in l:
+            l = re.sub(r"[-+0-9.eE]+(\s*)$", r"T\1", l)
+        out.append(l)
+    return out
+
+
+@pytest.mark.parametrize("mode", [[], ["--batched", "--fp32"]])
+def test_cli_stdout_matches_reference_restatement(goldens, tmp_path, mode):
+    """betc's stdout, line for line, against the reference's stdout restated
+    independently: ' n_bcc=', one 'BCC number #:' line per block of more
+    than two vertices in Tarjan emission order with the arc count, the
+    per-block lines, the active / free totals, timings blanked.  Inputs: the
+    four sample inputs and every other golden graph (multi-block graphs with
+    bridges, chains and pendants among them), written in file order."""
+    exe = os.path.join(bcx.HERE, "betc")
+    for name, g in goldens.items():
+        src = tmp_path / f"{name}.txt"
+        _write_edges(src, g["n"], [tuple(e) for e in g["edges"]])
+        out = tmp_path / f"{name}.bc"
+        r = subprocess.run([exe, str(src), "--bc-out-exact", str(out)] + mode, capture_output=True, text=True,
+                           cwd=tmp_path, timeout=120)
+        assert r.returncode == 0, (name, r.stderr)
+        assert _blank(r.stdout) == _expected_stdout(g["reference_structure"]), name
+        mx, _, zeros = relerr(np.loadtxt(out, ndmin=2)[:, 1], g["bc"])
+        assert mx < (1e-6 if mode else BC_TOL) and zeros, name
+
+
 def test_cli_batched_fp32(goldens, tmp_path):
     exe = os.path.join(bcx.HERE, "betc")
     src = os.path.join(bcx.HERE, "..", "tests", "golden", "test4.txt")

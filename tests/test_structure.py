@@ -32,6 +32,34 @@ def test_tarjan_order_sample1():
     assert list(comps[0].gid) == seen
 
 
+def test_emission_order_local_ids_and_counts_vs_restatement(goldens):
+    """Every golden graph (the reference's four sample inputs, multi-block
+    graphs with bridges, chains and pendants): BCC sizes in emission order,
+    the local ids of every component and the active / free counts equal an
+    independent recursive restatement of the reference's Tarjan, relabel and
+    ear bookkeeping (tests/golden/make_goldens.py: reference_structure)."""
+    multi = 0
+    for name, g in goldens.items():
+        ref = g["reference_structure"]
+        h = bcx.Host.from_edges(g["n"], g["edges"]).decompose()
+        assert h.num_bcc() == len(ref["bcc_sizes"]), name
+        for i, (nv, ne) in enumerate(ref["bcc_sizes"]):
+            assert h.bcc_size(i) == (nv, ne), (name, i)
+        big = [i for i, (nv, _) in enumerate(ref["bcc_sizes"]) if nv > 2]
+        comps = h.components()
+        assert [c.bcc for c in comps] == big, name
+        for c in comps:
+            assert list(c.gid) == ref["local_ids"][c.bcc], (name, c.bcc)
+        active = free = 0
+        for k in range(len(comps)):
+            f, e, _ = h.component_counts(k)
+            active += e
+            free += f
+        assert ref["lines"][-1] == f"active_vert_count {active} free_vert_count {free} ", name
+        multi += len(big) > 1
+    assert multi >= 2
+
+
 def test_cut_term_and_reach_vs_bruteforce(goldens):
     for name, g in goldens.items():
         n = g["n"]
