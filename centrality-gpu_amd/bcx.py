@@ -97,6 +97,8 @@ SIGNATURES = {
     "bcx_debug_tile": (C.c_int, [C.c_void_p, C.POINTER(Component), _i32p, C.c_int32, C.POINTER(Opts),
                                  _i32p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                  C.POINTER(C.c_double), _i32p, _i32p]),
+    "bcx_bc_small_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "bcx_generate": (C.c_int, [C.c_int32, C.c_uint64, C.c_double, _i32p, _i64p,
                                C.POINTER(_i32p)]),
     "bcx_batched_order": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _i32p]),

@@ -130,6 +130,10 @@ int64_t row_capacity(bcx_ctx *ctx, int32_t n, const bcx_opts *o);
 int plain_component(bcx_ctx *ctx, const bcx::DComp &dc, const bcx::Plan &plan, const bcx_opts *o,
                     double *bc, bcx_stats *st);
 
+// Device address of the no-allocation entry point's static scratch array on
+// `device` (current), looked up once per device (bcx_noalloc.hip).
+int noalloc_init(int device);
+
 // batched path (bcx_batched_host.hip)
 struct SmallJob {
     const bcx::DComp *dc;

@@ -438,6 +438,9 @@ int bcx_create(bcx_ctx **out, int device, size_t mem_cap) {
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     ctx->mem_cap = mem_cap ? mem_cap : (size_t)(0.9 * (double)free_b);
+    // a failed lookup is recorded there and reported by bcx_bc_small_noalloc;
+    // every other entry point works without the array
+    (void)noalloc_init(device);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return BCX_E_HIP;

@@ -47,6 +47,9 @@ struct SmallArgs {
     // level and the dependency of every (vertex, lane) of the last tile
     int32_t *lvl;
     double *dbg;
+    // when set, the tile count is read here at kernel start (written by an
+    // earlier kernel on the stream) instead of n_tiles
+    const int32_t *n_tiles_dev;
 };
 
 hipError_t launch_small(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st);

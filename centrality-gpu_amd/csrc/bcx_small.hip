@@ -57,11 +57,12 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
     uint64_t *Qm = a.Qm + (int64_t)blockIdx.x * a.slab;
     int32_t *plv = a.lvl ? a.lvl + (int64_t)blockIdx.x * a.slab : nullptr;
     double *pdl = a.dbg ? a.dbg + (int64_t)blockIdx.x * a.slab : nullptr;
+    const int n_tiles = a.n_tiles_dev ? *a.n_tiles_dev : a.n_tiles;
     while (true) {
         if (tid == 0) s_item = atomicAdd(&a.counter[0], 1);
         __syncthreads();
         const int item = s_item;
-        if (item >= a.n_tiles) break;
+        if (item >= n_tiles) break;
         const SmallTile tl = a.tiles[item];
         const SmallComp cp = a.comps[tl.comp];
         const int n = cp.n;

@@ -222,6 +222,31 @@ int  bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts,
 int  bcx_bc(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps,
             const bcx_opts *opts, double *bc_out, bcx_stats *st);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the batched run (bcx_run, BCX_PATH_BATCHED, default
+ * flags) of ONE component of at most BCX_NOALLOC_MAX_N vertices -- the
+ * small-component kernel -- that allocates nothing.  Every buffer beyond the
+ * inputs and the output (tile table, source weights, work counters, and the
+ * path-count / term / list slabs of at most BCX_NOALLOC_MAX_WORKGROUPS
+ * workgroups) lives in one static device array of BCX_NOALLOC_SCRATCH_BYTES
+ * per GPU, held from the first bcx_create on that GPU until the process
+ * ends.  Inputs and output are the caller's device memory: d_row_ptr[n+1],
+ * d_col[arcs], d_reach[n] (NULL = all zero), d_srcs[n_src] (distinct local
+ * vertices; NULL = every vertex, n_src ignored); d_bc[n] is ADDED to, in
+ * the FP64 convention of bcx_run.  precision is BCX_PREC_*.  The call runs
+ * on the context's stream and returns after the work is done.
+ * BCX_E_ARG when n > BCX_NOALLOC_MAX_N, n_src > n or a source is out of
+ * range (then nothing is added).  The scratch array keeps the last call's
+ * contents and is shared by every context on that GPU: calls that overlap
+ * on one GPU race and must be serialised by the caller.                    */
+#define BCX_NOALLOC_MAX_N           2048
+#define BCX_NOALLOC_MAX_WORKGROUPS  8
+#define BCX_NOALLOC_SCRATCH_BYTES   (28u * 1024u * 1024u + 64u * 1024u)
+int  bcx_bc_small_noalloc(bcx_ctx *ctx, int32_t n, int64_t arcs, const int32_t *d_row_ptr,
+                          const int32_t *d_col, const int32_t *d_reach, const int32_t *d_srcs,
+                          int32_t n_src, int32_t precision, double *d_bc);
+
 /* Whole pipeline for a decomposed host graph: GPU BC of every component
  * scattered to global ids plus the cut-vertex term (the reference's
  * bc_result, Codes/to_eargraph_construct.cpp:166-179 without the double
