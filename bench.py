@@ -122,7 +122,7 @@ def summarize_trace(path):
     for k, iv in sorted(rows.items(), key=lambda kv: -tot(kv[1])):
         print(f"| `{k[:60]}` | {len(iv)} | {tot(iv) / 1e6:.1f} | {_union(iv) / 1e6:.1f} | "
               f"{tot(iv) / len(iv) / 1e3:.1f} |")
-    b = [x for k, iv in rows.items() if k.startswith(BATCHED_KERNELS) for x in iv]
+    b = [x for k, iv in rows.items() if any(p in k for p in BATCHED_KERNELS) for x in iv]
     if b:
         print(f"\nBatched kernels: {len(b)} launches, union {_union(b) / 1e6:.1f} ms, "
               f"span first start to last end {(max(e for _, e in b) - min(s for s, _ in b)) / 1e6:.1f} ms")
