@@ -55,7 +55,7 @@ class Stats(C.Structure):
                 ("n_bfs_sources", C.c_int64), ("n_chain_sources", C.c_int64),
                 ("n_row_only", C.c_int64), ("n_f64_sigma", C.c_int64), ("n_launches", C.c_int64),
                 ("max_level", C.c_int32), ("n_ball_tiles", C.c_int32), ("bytes_alg", C.c_double),
-                ("teps_work", C.c_double), ("t_chain_s", C.c_double)]
+                ("teps_work", C.c_double), ("t_chain_s", C.c_double), ("n_graph_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -34,8 +34,11 @@ struct BatchDev {
     uint64_t *hub_mask;               // [n_hub * hub_stride] lanes of hub at level l
     double *hub_acc;                  // [n_hub * 64] backward partial sums
     int32_t *hub_cnt;                 // [n_hub] slices finished this level
-    const int32_t *srcs;              // [64] source per lane
-    const double *ws;                 // [64] source weight r(s)+1 per lane
+    const int32_t *srcs;              // [64] source per lane (read by the seed kernel only)
+    const double *ws;                 // [64] source weight r(s)+1 per lane (seed kernel only)
+    double *wsl;                      // [64] the slot's copy of ws, made by the seed kernel:
+                                      // the level kernels read it, so their launches are the
+                                      // same for every tile of the slot (replayable graph)
     double *bc;                       // the slot's BC accumulator (folded per component)
     uint64_t limit;                   // integer sigma threshold (0 = the word's range)
 };

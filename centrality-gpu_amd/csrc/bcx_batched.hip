@@ -111,6 +111,7 @@ __global__ __launch_bounds__(BB) void k_b_clear_vis(BatchDev b, int sm) {
 template <int SM>
 __global__ void k_b_seed(BatchDev b, int ns) {
     const int k = threadIdx.x;
+    b.wsl[k] = k < ns ? b.ws[k] : 0.0;  // (launched with 64 threads)
     if (k < ns) {
         const int s = b.srcs[k];
         const uint64_t bit = 1ull << k;
@@ -406,7 +407,7 @@ __device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t
     if (M & (1ull << lane)) {
         const double sv = sg_val<SM>(sg<SM>(b, (int64_t)v * 64 + lane));
         const double delta = sv * acc;
-        contrib = b.ws[lane] * delta;
+        contrib = b.wsl[lane] * delta;
         ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + acc);
         if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
     }
@@ -555,7 +556,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                 const double sv = sg_val<SM>(SX[j]);
                 const double a = r_rho[wv][j][lane];  // rho_v
                 const double delta = sv * a;
-                contrib = b.ws[lane] * delta;
+                contrib = b.wsl[lane] * delta;
                 ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
                 if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
             }

@@ -36,8 +36,21 @@ constexpr int HUB_MIN = 512, HUB_SLICES_PER_CU = 80, HUB_LEVELS = 1024;
 // Chung-Lu and R-MAT cores (6 equals 4, 12 and 16 equal 8).
 constexpr int MAX_SLOTS = 8;
 
+// A tile's level launches, captured once and replayed for every tile whose
+// launch arguments are the same: the arguments hold no tile index (the seed
+// kernel, launched per tile, copies the tile's sources and weights into the
+// slot), so a key covers every tile of a component on that slot with the
+// same count mode, level count and precision.
+struct TileGraph {
+    BatchDev key;
+    int sm = 0, C = 0, grid = 0;
+    bool fp32 = false;
+    hipGraphExec_t exec = nullptr;
+};
+constexpr int GRAPHS_PER_SLOT = 12;
+
 struct Slot {
-    DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, hub_mask, hub_acc, hub_cnt, lvl, dbg, bcacc;
+    DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, hub_mask, hub_acc, hub_cnt, lvl, dbg, bcacc, wsl;
     int32_t n_cap = 0, qcap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -45,6 +58,7 @@ struct Slot {
     // tile in flight
     int tile = -1, ns = 0, L = 0, C = 0, attempts = 0;
     int sm = SIG_U32;  // sigma mode of the tile in flight
+    std::vector<TileGraph> graphs;  // most recently used last
 };
 
 struct BatchedState {
@@ -93,7 +107,8 @@ static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
             return rc;
         s.qcap = (int32_t)q;
     }
-    if ((rc = s.ctr.ensure(ctx, 64, "batched counters"))) return rc;
+    if ((rc = s.ctr.ensure(ctx, 64, "batched counters")) || (rc = s.wsl.ensure(ctx, 8 * BATCH_W, "tile weights")))
+        return rc;
     if (probe && ((rc = s.lvl.ensure(ctx, 4 * 64 * N, "probe levels")) ||
                   (rc = s.dbg.ensure(ctx, 8 * 64 * N, "probe delta"))))
         return rc;
@@ -168,6 +183,7 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uin
     b.hub_cnt = (int32_t *)s.hub_cnt.p;
     b.srcs = (const int32_t *)bs.srcs.p + (size_t)s.tile * BATCH_W;
     b.ws = (const double *)bs.ws.p + (size_t)s.tile * BATCH_W;
+    b.wsl = (double *)s.wsl.p;
     b.bc = (double *)s.bcacc.p;
     (void)bc;
     b.limit = limit;
@@ -181,20 +197,75 @@ struct TileJob {
     uint64_t limit;
     bool fp32;
     int grid;
+    bool graphs;  // replay captured level launches (first chunk of a tile)
 };
 
 // Forward levels [L, L+C), the speculative backward L+C-1 .. 1 (kernels past
 // the depth or of an unfinished forward return at once), the cleanup, the
 // counter read-back and the completion event.
+static int launch_levels(bcx_ctx *ctx, Slot &s, const BatchDev &b, int L0, int C, const TileJob &j) {
+    int rc;
+    for (int l = L0; l < L0 + C; ++l)
+        if ((rc = ctx_hip(ctx, launch_b_forward(b, s.sm, l, j.grid, s.stream), "forward"))) return rc;
+    for (int l = L0 + C - 1; l >= 1; --l)
+        if ((rc = ctx_hip(ctx, launch_b_back(b, s.sm, j.fp32, l, j.grid, s.stream), "back"))) return rc;
+    return ctx_hip(ctx, launch_b_finish(b, j.grid, s.stream), "finish");
+}
+
+// The first chunk's launches from the slot's graph cache (captured on a miss).
+static int replay_levels(bcx_ctx *ctx, Slot &s, BatchDev b, const TileJob &j) {
+    b.srcs = nullptr;  // read by the seed kernel only: not part of the key
+    b.ws = nullptr;
+    auto hit = std::find_if(s.graphs.begin(), s.graphs.end(), [&](const TileGraph &g) {
+        return g.sm == s.sm && g.C == s.C && g.grid == j.grid && g.fp32 == j.fp32 &&
+               !std::memcmp(&g.key, &b, sizeof(b));
+    });
+    if (hit == s.graphs.end()) {
+        if ((int)s.graphs.size() >= GRAPHS_PER_SLOT) {
+            (void)hipGraphExecDestroy(s.graphs.front().exec);
+            s.graphs.erase(s.graphs.begin());
+        }
+        TileGraph g;
+        std::memcpy(&g.key, &b, sizeof(b));
+        g.sm = s.sm;
+        g.C = s.C;
+        g.grid = j.grid;
+        g.fp32 = j.fp32;
+        hipGraph_t graph = nullptr;
+        int rc = ctx_hip(ctx, hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "capture");
+        if (rc) return rc;
+        const int rl = launch_levels(ctx, s, b, 0, s.C, j);
+        const hipError_t ee = hipStreamEndCapture(s.stream, &graph);
+        if (rl || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rl ? rl : ctx_hip(ctx, ee, "capture");
+        }
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return ctx_hip(ctx, ei, "graph instantiate");
+        s.graphs.push_back(g);
+        hit = s.graphs.end() - 1;
+    } else if (hit != s.graphs.end() - 1) {
+        std::rotate(hit, hit + 1, s.graphs.end());  // most recently used last
+        hit = s.graphs.end() - 1;
+    }
+    return ctx_hip(ctx, hipGraphLaunch(hit->exec, s.stream), "graph launch");
+}
+
+// Forward levels [L, L+C), the speculative backward L+C-1 .. 1 (kernels past
+// the depth or of an unfinished forward return at once), the cleanup, the
+// counter read-back and the completion event.  A tile's first chunk replays a
+// captured graph; a deeper tile's later chunks are launched one by one.
 static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st) {
     BatchDev b = make_dev(ctx, s, *j.dc, j.bc, s.sm == SIG_F64 ? 0 : j.limit);
     int rc;
-    for (int l = s.L; l < s.L + s.C; ++l)
-        if ((rc = ctx_hip(ctx, launch_b_forward(b, s.sm, l, j.grid, s.stream), "forward"))) return rc;
+    if (j.graphs && s.L == 0) {
+        if ((rc = replay_levels(ctx, s, b, j))) return rc;
+        if (st) st->n_graph_launches++;
+    } else if ((rc = launch_levels(ctx, s, b, s.L, s.C, j))) {
+        return rc;
+    }
     s.L += s.C;
-    for (int l = s.L - 1; l >= 1; --l)
-        if ((rc = ctx_hip(ctx, launch_b_back(b, s.sm, j.fp32, l, j.grid, s.stream), "back"))) return rc;
-    if ((rc = ctx_hip(ctx, launch_b_finish(b, j.grid, s.stream), "finish"))) return rc;
     if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1;
     if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.pinned, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream),
                       "counters")))
@@ -207,7 +278,9 @@ static int start_tile(bcx_ctx *ctx, Slot &s, int tile, int ns, int sm, const Til
     s.ns = ns;
     s.sm = sm;
     s.L = 0;
-    s.C = std::max(4, ctx->batched->last_depth + 3);
+    // levels of the first chunk: the last tile's depth + 3, rounded up to a
+    // multiple of 8 so that a handful of captured graphs serve every tile
+    s.C = (std::max(4, ctx->batched->last_depth + 3) + 7) & ~7;
     BatchDev b = make_dev(ctx, s, *j.dc, j.bc, sm == SIG_F64 ? 0 : j.limit);
     int rc = ctx_hip(ctx, launch_b_init(b, ns, sm, j.grid, s.stream), "init");
     if (rc) return rc;
@@ -466,6 +539,7 @@ static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double
                   ? (1ull << o->sigma_f64_threshold_log2) : 0;
     j.fp32 = o->precision == BCX_PREC_FP32C;
     j.grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)dc.n + 3) / 4, (int64_t)ctx->n_cu * 8));
+    j.graphs = !(o->flags & BCX_FLAG_NO_GRAPH);
     return j;
 }
 
@@ -795,7 +869,7 @@ int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs,
 // small components: one persistent launch over (component, 64-source) tiles
 // ------------------------------------------------------------------------
 struct SmallState {
-    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs, lvl, dbg;
+    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs, qr, ro, lvl, dbg, psig;
     int grid = 0;
     // a pass started and not yet finished (persistent_start / _finish)
     bool pending = false, run_fp32 = false, run_graph = false;
@@ -813,11 +887,15 @@ struct SmallState {
 };
 
 // Reset + persistent kernel, replayed from a captured graph when possible.
-static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool f64, bool fp32, int grid,
+static hipError_t launch_pass(const SmallArgs &a, bool wg, bool f64, bool fp32, int grid, hipStream_t sp) {
+    return wg ? launch_wg(a, f64, fp32, grid, sp) : launch_small(a, f64, fp32, grid, sp);
+}
+
+static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool wg, bool f64, bool fp32, int grid,
                         bool use_graph, hipStream_t sp) {
     if (!use_graph) {
-        int rc = ctx_hip(ctx, launch_small_reset(a.counter, sp), "small reset");
-        return rc ? rc : ctx_hip(ctx, launch_small(a, f64, fp32, grid, sp), "small");
+        int rc = ctx_hip(ctx, launch_small_reset(a.counter, sp), "persistent reset");
+        return rc ? rc : ctx_hip(ctx, launch_pass(a, wg, f64, fp32, grid, sp), "persistent kernel");
     }
     const bool same = ss.exec && !std::memcmp(&ss.key, &a, sizeof(a)) && ss.key_grid == grid &&
                       ss.key_f64 == f64 && ss.key_fp32 == fp32;
@@ -828,7 +906,7 @@ static int small_launch(bcx_ctx *ctx, SmallState &ss, const SmallArgs &a, bool f
         int rc = ctx_hip(ctx, hipStreamBeginCapture(sp, hipStreamCaptureModeThreadLocal), "capture");
         if (rc) return rc;
         hipError_t e1 = launch_small_reset(a.counter, sp);
-        hipError_t e2 = launch_small(a, f64, fp32, grid, sp);
+        hipError_t e2 = launch_pass(a, wg, f64, fp32, grid, sp);
         hipError_t e3 = hipStreamEndCapture(sp, &graph);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -859,10 +937,7 @@ static int follow_ctx_stream(bcx_ctx *ctx, hipStream_t sp) {
 }
 
 static int persistent_pass(bcx_ctx *ctx, SmallState &ss, bool wg, bool f64, bool graph) {
-    const SmallArgs &a = ss.run;
-    if (!wg) return small_launch(ctx, ss, a, f64, ss.run_fp32, ss.run_grid, graph, ss.run_stream);
-    int r = ctx_hip(ctx, launch_small_reset(a.counter, ss.run_stream), "workgroup reset");
-    return r ? r : ctx_hip(ctx, launch_wg(a, f64, ss.run_fp32, ss.run_grid, ss.run_stream), "workgroup kernel");
+    return small_launch(ctx, ss, ss.run, wg, f64, ss.run_fp32, ss.run_grid, graph, ss.run_stream);
 }
 
 // One persistent launch over the tiles of `jobs`: the small-component kernel
@@ -896,6 +971,9 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
         const SmallJob &jb = *jbp;
         const DComp &dc = *jb.dc;
         if (!wide(jbp)) n_narrow_tiles += (jb.srcs.size() + BATCH_W - 1) / BATCH_W;
+        // (the workgroup kernel stays on the component's own ids: on the
+        // BFS-ordered copy the 256x256 grid workload fetched 32% more bytes
+        // and ran 8% slower)
         SmallComp c{};
         c.n = dc.n;
         c.rp = dc.dev.rp;
@@ -930,7 +1008,7 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     const size_t slab = (size_t)(wg ? nmax : SMALL_N) * 64;
     // list entries: a vertex joins at most one level per lane, so 64 n bounds a tile
     const size_t qcap = wg ? (size_t)nmax * 64 + 64 : slab;
-    const size_t per_wg = 16 * slab + 12 * qcap + (wg ? 24 * (size_t)nmax + 4 * ((size_t)nmax + 2) : 0);
+    const size_t per_wg = 16 * slab + 12 * qcap + (wg ? 4 * qcap + 32 * (size_t)nmax + 4 * ((size_t)nmax + 2) : 0);
     int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), (wg ? 1 : 2) * (int64_t)ctx->n_cu));
     if (wg) {
         const size_t room = ctx->mem_cap > ctx->mem_used ? (ctx->mem_cap - ctx->mem_used) / 2 : 0;
@@ -942,8 +1020,11 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
         (rc = ss.Qm.ensure(ctx, 8 * qcap * grid, "persistent lists")))
         return rc;
     if (wg && ((rc = ss.masks.ensure(ctx, 24 * (size_t)nmax * grid, "workgroup masks")) ||
+               (rc = ss.qr.ensure(ctx, 4 * qcap * grid, "workgroup record offsets")) ||
+               (rc = ss.ro.ensure(ctx, 8 * (size_t)nmax * grid, "workgroup record offsets")) ||
                (rc = ss.qoffs.ensure(ctx, 4 * ((size_t)nmax + 2) * grid, "workgroup level offsets"))))
         return rc;
+    if (probe && wg && (rc = ss.psig.ensure(ctx, 8 * slab * grid, "probe path counts"))) return rc;
     if (probe && ((rc = ss.lvl.ensure(ctx, 4 * slab * grid, "probe levels")) ||
                   (rc = ss.dbg.ensure(ctx, 8 * slab * grid, "probe delta"))))
         return rc;
@@ -966,11 +1047,14 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     a.qoffs = wg ? (int32_t *)ss.qoffs.p : nullptr;
     a.nmax = nmax;
     a.qcap = (int64_t)qcap;
+    a.qr = wg ? (int32_t *)ss.qr.p : nullptr;
+    a.ro = wg ? (int32_t *)ss.ro.p : nullptr;
+    a.psig = probe && wg ? (uint64_t *)ss.psig.p : nullptr;
     a.lvl = probe ? (int32_t *)ss.lvl.p : nullptr;
     a.dbg = probe ? (double *)ss.dbg.p : nullptr;
     ss.run = a;
     ss.run_fp32 = o->precision == BCX_PREC_FP32C;
-    ss.run_graph = !wg && !(o->flags & BCX_FLAG_NO_GRAPH);
+    ss.run_graph = !(o->flags & BCX_FLAG_NO_GRAPH);
     ss.run_tiles = std::move(tiles);
     ss.run_narrow = n_narrow_tiles;
     ss.run_grid = (int)grid;
@@ -983,7 +1067,10 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
         ctx->timer.begin(ctx);
         if ((rc = persistent_pass(ctx, ss, wg, false, ss.run_graph))) return rc;
         ctx->timer.end(ctx);
-        if (st) st->n_launches += 2;
+        if (st) {
+            st->n_launches += 2;
+            if (ss.run_graph) st->n_graph_launches++;
+        }
     }
     return BCX_OK;
 }
@@ -1046,7 +1133,7 @@ static void release_state(bcx_ctx *ctx, SmallState *&state) {
     SmallState &ss = *state;
     if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
     for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm,
-                    &ss.masks, &ss.qoffs, &ss.lvl, &ss.dbg})
+                    &ss.masks, &ss.qoffs, &ss.qr, &ss.ro, &ss.lvl, &ss.dbg, &ss.psig})
         b->release(ctx);
     delete state;
     state = nullptr;
@@ -1093,21 +1180,28 @@ int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *s
     }
     if (!rc && kind != BCX_KERNEL_MULTI) {
         const bool wg = kind == BCX_KERNEL_WG;
-        for (int32_t v = 0; v < n; ++v) at[v] = v;
-        std::vector<SmallJob> jobs{{&dc, std::vector<int32_t>(srcs, srcs + ns)}};
         bcx_stats st{};
-        rc = persistent_start(ctx, jobs, o, bc, &st, wg, ctx->stream, true);
-        if (!rc) rc = persistent_finish(ctx, &st, wg);
+        if (!rc) {
+            for (int32_t v = 0; v < n; ++v) at[v] = v;
+            std::vector<SmallJob> jobs{{&dc, std::vector<int32_t>(srcs, srcs + ns)}};
+            rc = persistent_start(ctx, jobs, o, bc, &st, wg, ctx->stream, true);
+            if (!rc) rc = persistent_finish(ctx, &st, wg);
+        }
         f64 = st.n_f64_sigma > 0;
         // one tile, so one workgroup: its slab is the start of every array
-        const SmallState &ss = *(wg ? ctx->wgk : ctx->small);
-        if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), ss.lvl.p, 4 * cells, hipMemcpyDeviceToHost, ctx->stream),
-                              "copy");
-        if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), ss.sig.p, 8 * cells, hipMemcpyDeviceToHost, ctx->stream),
-                              "copy");
-        if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), ss.dbg.p, 8 * cells, hipMemcpyDeviceToHost, ctx->stream),
-                              "copy");
-        if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "copy");
+        // (the workgroup kernel keeps lane-compacted records: its probe writes
+        // the path counts per (vertex, lane) beside them)
+        const SmallState *ss = wg ? ctx->wgk : ctx->small;
+        if (!rc && ss) {
+            const void *sg_src = wg ? ss->psig.p : ss->sig.p;
+            rc = ctx_hip(ctx, hipMemcpyAsync(lv.data(), ss->lvl.p, 4 * cells, hipMemcpyDeviceToHost, ctx->stream),
+                         "copy");
+            if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(sg.data(), sg_src, 8 * cells, hipMemcpyDeviceToHost,
+                                                      ctx->stream), "copy");
+            if (!rc) rc = ctx_hip(ctx, hipMemcpyAsync(dl.data(), ss->dbg.p, 8 * cells, hipMemcpyDeviceToHost,
+                                                      ctx->stream), "copy");
+            if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "copy");
+        }
     } else if (!rc) {
         DComp *pc = nullptr;
         rc = relabelled(ctx, dc, pc);
@@ -1168,8 +1262,10 @@ void batched_release(bcx_ctx *ctx) {
     BatchedState &bs = *ctx->batched;
     for (Slot &s : bs.slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
+        for (TileGraph &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
+        s.graphs.clear();
         for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.qoff, &s.hub_mask, &s.hub_acc,
-                        &s.hub_cnt, &s.lvl, &s.dbg, &s.bcacc})
+                        &s.hub_cnt, &s.lvl, &s.dbg, &s.bcacc, &s.wsl})
             b->release(ctx);
         if (s.pinned) (void)hipHostFree(s.pinned);
         if (s.done) (void)hipEventDestroy(s.done);

@@ -653,9 +653,9 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         // larger components: tile order from the tuner (BFS ranks or balls)
         std::vector<int32_t> tiles = tuned_sources(dc, srcs);
         if (dc.tile_order == 1) s->n_ball_tiles++;
-        if (kind == BCX_KERNEL_WG)
+        if (kind == BCX_KERNEL_WG) {
             wg.push_back({&dc, std::move(tiles)});
-        else {
+        } else {
             if ((rc = batched_prepare(ctx, dc))) return rc;  // one-time work stays outside the timing
             multi.push_back({&dc, std::move(tiles)});
         }

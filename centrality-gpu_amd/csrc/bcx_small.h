@@ -43,6 +43,11 @@ struct SmallArgs {
     int32_t *qoffs;     // [grid][nmax + 2]
     int64_t nmax;       // largest component
     int64_t qcap;       // list entries per workgroup (64 * nmax + 64)
+    // workgroup-per-tile kernel: sig / tt hold lane-compacted records (one
+    // word per lane of each list entry) located through these offsets
+    int32_t *qr;        // [grid][qcap] record offset of each list entry
+    int32_t *ro;        // [grid][2 * nmax] record offset of each vertex's current entry
+    uint64_t *psig;     // probe only: [grid][slab] path counts per (vertex, lane)
     // probe only (null in runs): per-workgroup slabs like sig, holding the
     // level and the dependency of every (vertex, lane) of the last tile
     int32_t *lvl;

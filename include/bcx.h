@@ -161,8 +161,9 @@ typedef struct {
  * multi-launch tile path instead of the one-launch small-component kernel
  * (both compute the same; the flag exists for tests and measurements).     */
 #define BCX_FLAG_NO_SMALL_KERNEL 1
-/* Batched path: launch the small-component kernel eagerly instead of
- * replaying its captured hipGraph.                                         */
+/* Batched path: launch every kernel eagerly instead of replaying captured
+ * hipGraphs (the persistent small-component and workgroup passes, and the
+ * level launches of each multi-launch tile).                               */
 #define BCX_FLAG_NO_GRAPH        2
 /* Batched path: components of 2049..131072 vertices whose BFS levels are
  * narrow (average level width at most 4096 vertices from the highest-degree
@@ -186,6 +187,8 @@ typedef struct {
     double  bytes_alg;         /* algorithmic HBM bytes of the work done      */
     double  teps_work;         /* sum over sources of component arcs/2        */
     double  t_chain_s;         /* part of t_kernel_s spent on chain sources   */
+    int64_t n_graph_launches;  /* batched: replays of captured launch sequences
+                                  (their kernels are counted in n_launches)   */
 } bcx_stats;
 
 /* Context: device, streams, scratch.  mem_cap (bytes, 0 = 90% of free

@@ -235,6 +235,35 @@ def test_small_kernel_graph_replays(engine):
         g.free()
 
 
+@pytest.mark.parametrize("kernel_flags", [bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_NO_WG_KERNEL,
+                                          bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_WG_KERNEL])
+def test_tile_graph_replays(engine, kernel_flags):
+    """The captured level launches of multi-launch tiles (one graph per slot,
+    count mode and level count) and the captured workgroup-kernel pass,
+    replayed over several runs with eager work and synchronisations between
+    them, equal the eager launches every time and the oracle."""
+    n, uv = bcx.generate(2, 0, 0.01)
+    core = max(comps_of(n, uv), key=lambda c: c.n)
+    want = oracle.component_bc(core.row_ptr, core.col, core.reach)
+    g = engine.upload([core])
+    try:
+        eager, st = g.run(path=B, precision=bcx.PREC_FP64, flags=kernel_flags | bcx.FLAG_NO_GRAPH)
+        assert st["n_graph_launches"] == 0
+        mx, _, zeros = relerr(eager, want)
+        assert mx < 1e-9 and zeros
+        for rep in range(4):
+            got, st = g.run(path=B, precision=bcx.PREC_FP64, flags=kernel_flags)
+            assert st["n_graph_launches"] > 0, rep
+            np.testing.assert_allclose(got, eager, rtol=1e-12, atol=0, err_msg=f"replay {rep}")
+            other, _ = g.run(path=bcx.PATH_PLAIN, precision=bcx.PREC_FP64)  # eager work between replays
+            np.testing.assert_allclose(other, eager, rtol=1e-9, atol=0)
+        got, st = g.run(path=B, precision=bcx.PREC_FP32C, flags=kernel_flags)
+        mx, _, zeros = relerr(got, want)
+        assert mx < 1e-6 and zeros
+    finally:
+        g.free()
+
+
 @pytest.mark.parametrize("flags", [0, bcx.FLAG_NO_SMALL_KERNEL])
 def test_fp32c_sigma_beyond_f32_range(engine, flags):
     """Layers of 3 vertices, consecutive layers completely joined: sigma
