@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--summarize-trace", default="",
                     help="rocprofv3 kernel-trace CSV: print per-kernel calls, summed and union busy time")
     ap.add_argument("--slots", type=int, default=0, help="concurrent tiles of the batched path (0 = library default)")
+    ap.add_argument("--flags", type=int, default=0,
+                    help="BCX_FLAG_* bits of the batched run, for A/B measurements (16 = no chain contraction)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -232,7 +234,8 @@ def main():
     K = args.steps
     idx = [i % args.slices for i in range(K)]
     t_b, tk_b, st_b = timed_steps(g, slices, idx, args.warmup, path=bcx.PATH_BATCHED,
-                                  precision=bcx.PREC_FP32C, use_chains=True, batch_width=args.slots)
+                                  precision=bcx.PREC_FP32C, use_chains=True, batch_width=args.slots,
+                                  flags=args.flags)
     src_b = sum(len(slices[i][0]) for i in idx)
     work_b = W * src_b / total_src
     mteps_b = work_b / t_b / 1e6

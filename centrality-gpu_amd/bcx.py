@@ -55,7 +55,8 @@ class Stats(C.Structure):
                 ("n_bfs_sources", C.c_int64), ("n_chain_sources", C.c_int64),
                 ("n_row_only", C.c_int64), ("n_f64_sigma", C.c_int64), ("n_launches", C.c_int64),
                 ("max_level", C.c_int32), ("n_ball_tiles", C.c_int32), ("bytes_alg", C.c_double),
-                ("teps_work", C.c_double), ("t_chain_s", C.c_double), ("n_graph_launches", C.c_int64)]
+                ("teps_work", C.c_double), ("t_chain_s", C.c_double), ("n_graph_launches", C.c_int64),
+                ("n_contracted", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -310,6 +311,7 @@ FLAG_NO_SMALL_KERNEL = 1
 FLAG_NO_GRAPH = 2
 FLAG_NO_WG_KERNEL = 4
 FLAG_WG_KERNEL = 8
+FLAG_NO_CONTRACT = 16
 KERNEL_SMALL, KERNEL_WG, KERNEL_MULTI = 1, 2, 3
 
 

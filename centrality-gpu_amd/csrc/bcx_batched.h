@@ -21,7 +21,8 @@ struct BatchDev {
     int32_t *Q;                       // level lists, concatenated
     uint64_t *Qm;                     // lane mask of each list entry
     int32_t *ctr;                     // [0] list end, [1] sigma overflow, [2] list overflow,
-                                      // [3] depth (-1 while the forward runs)
+                                      // [3] depth (-1 while the forward runs), [4] last
+                                      // injection level, [5] sources in the tile
     int32_t *qoff;                    // level offsets into Q (device-maintained)
     int32_t qcap;
     // hub vertices (degree > hub_deg): their arcs are cut into static slices
@@ -41,6 +42,25 @@ struct BatchDev {
                                       // same for every tile of the slot (replayable graph)
     double *bc;                       // the slot's BC accumulator (folded per component)
     uint64_t limit;                   // integer sigma threshold (0 = the word's range)
+    // Contracted chains (n_chain == 0: none).  Vertices [0, n) form the tile
+    // graph; the interiors of chain c are the ids n + ch_off[c] + i - 1,
+    // i = 1 .. K_c - 1, K_c = ch_off[c+1] - ch_off[c] + 1 arcs from L to R.
+    // An endpoint's row keeps one arc per attached chain, at the position of
+    // its arc into the chain, encoded as ~c (< 0): the level
+    // kernels skip it, and the pull adds a chain-source lane's own path (1)
+    // there, in CSR order.
+    int32_t n_chain;
+    const int32_t *ch_L, *ch_R;       // [n_chain] endpoints
+    const int32_t *ch_off;            // [n_chain + 1] first interior of each chain
+    const double *ch_pre;             // [interiors] P_c(i) = sum over interiors 1..i of r(x)+1
+    const int32_t *ch_of;             // [interiors] chain of each interior
+    const int32_t *ep_idx;            // [n] endpoint index of a vertex (-1: none)
+    const int32_t *ep_off;            // [n_ep + 1] chains attached to endpoint e:
+    const int32_t *ep_list;           //   ep_list[ep_off[e] ..) = 2c + (0: e is L, 1: R)
+    int32_t *ep_lvl;                  // [n_ep * 64] the tile's level of each endpoint per lane
+    int32_t *lane_ch;                 // [2 * 64] per lane: chain of its source (-1: a tile
+                                      // vertex) and the source's position in it
+    uint64_t *lane_mask;              // lanes whose source is a chain interior
 };
 
 // Sigma word of a tile: 32-bit integers, 64-bit integers, or FP64 summed in
@@ -54,6 +74,11 @@ hipError_t launch_b_init(const BatchDev &b, int ns, int sm, int cap, hipStream_t
 hipError_t launch_b_forward(const BatchDev &b, int sm, int l, int grid, hipStream_t st);
 hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st);
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st);
+// Contracted chains: the count check between forward and backward (a tie
+// interior's count sigma_L + sigma_R must fit the tile's word), and the
+// interiors' dependencies and BC after the backward.
+hipError_t launch_b_chain_check(const BatchDev &b, int sm, int grid, hipStream_t st);
+hipError_t launch_b_chains(const BatchDev &b, int sm, bool fp32, int grid, hipStream_t st);
 hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st);
 hipError_t launch_b_fold(double *dst, double *src, const int32_t *perm, int n, int grid, hipStream_t st);
 

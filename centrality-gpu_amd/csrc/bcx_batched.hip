@@ -108,26 +108,56 @@ __global__ __launch_bounds__(BB) void k_b_clear_vis(BatchDev b, int sm) {
     for (int64_t i = t0; i < b.n_hub; i += stride) b.hub_cnt[i] = 0;
 }
 
+// Lanes of a tile vertex start at level 0.  A lane whose source is the
+// interior x_j of a contracted chain (ids >= n) has no vertex at level 0: the
+// expand kernel injects it at L and at R, each with a path count of 1 (its
+// own path along the chain) on levels j and K - j, and the forward runs on
+// through empty levels while an injection into a vertex the lane has not yet
+// reached is pending.  (Shifting such a lane's levels down so that its nearer
+// endpoint enters at level 1 cuts a tile's depth but misaligns it with the
+// tile's other lanes, which the tile order groups by their true distances:
+// 115k against 123k MTEPS on the 1M-vertex mixed graph.)
 template <int SM>
 __global__ void k_b_seed(BatchDev b, int ns) {
-    const int k = threadIdx.x;
-    b.wsl[k] = k < ns ? b.ws[k] : 0.0;  // (launched with 64 threads)
-    if (k < ns) {
-        const int s = b.srcs[k];
+    const int k = threadIdx.x;  // (launched with 64 threads)
+    b.wsl[k] = k < ns ? b.ws[k] : 0.0;
+    int s = k < ns ? b.srcs[k] : -1;
+    int c = -1, j = 0, top = 0;
+    if (s >= b.n) {
+        const int x = s - b.n;
+        c = b.ch_of[x];
+        j = x - b.ch_off[c] + 1;
+        top = max(j, b.ch_off[c + 1] - b.ch_off[c] + 1 - j);
+        s = -1;
+    }
+    if (b.n_chain > 0) {
+        b.lane_ch[k] = c;
+        b.lane_ch[64 + k] = j;
+    }
+    const uint64_t core = __ballot(s >= 0);
+    const uint64_t chl = __ballot(c >= 0);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) top = max(top, __shfl_xor(top, o, 64));
+    if (s >= 0) {
         const uint64_t bit = 1ull << k;
+        const int pos = __popcll(core & (bit - 1));
         b.vis[s] = bit;
         b.mb0[s] = bit;
         sg_put<SM>(b, (int64_t)s * 64 + k, SM == SIG_F64 ? d2u(1.0) : 1ull);
-        b.Q[k] = s;
-        b.Qm[k] = bit;
+        b.Q[pos] = s;
+        b.Qm[pos] = bit;
         if (b.lvl) b.lvl[(int64_t)s * 64 + k] = 0;
+        if (b.n_chain > 0 && b.ep_idx[s] >= 0) b.ep_lvl[(int64_t)b.ep_idx[s] * 64 + k] = 0;
     }
     if (k == 0) {
-        b.ctr[0] = ns;
+        b.ctr[0] = __popcll(core);
         b.ctr[1] = 0;
         b.ctr[2] = 0;
         b.ctr[3] = -1;
+        b.ctr[4] = top;  // last level with an injected chain-source lane
+        b.ctr[5] = ns;
         b.qoff[0] = 0;
+        if (b.n_chain > 0) *b.lane_mask = chl;
     }
 }
 
@@ -135,6 +165,22 @@ __global__ void k_b_seed(BatchDev b, int ns) {
 __device__ __forceinline__ void push(const BatchDev &b, uint64_t *dst, int w, uint64_t F) {
     const uint64_t m = F & ~b.vis[w];
     if (m) atomicOr((unsigned long long *)&dst[w], (unsigned long long)m);  // result unused: no return
+}
+
+// Whether some chain-source lane still has an injection due after level l
+// into an endpoint it has not reached (vis is not written during an expand
+// launch, so every wave of the launch finds the same answer).
+__device__ __forceinline__ bool seed_pending(const BatchDev &b, int l) {
+    if (b.n_chain == 0) return false;
+    const int lane = threadIdx.x & 63;
+    const int c = b.lane_ch[lane];
+    bool p = false;
+    if (c >= 0) {
+        const int j = b.lane_ch[64 + lane], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
+        const uint64_t bit = 1ull << lane;
+        p = (j > l && !(b.vis[b.ch_L[c]] & bit)) || (K - j > l && !(b.vis[b.ch_R[c]] & bit));
+    }
+    return __ballot(p) != 0;
 }
 
 // Level l: Q_l = Q[qoff[l], ctr[0]).  Records qoff[l+1]; an empty Q_l ends the
@@ -145,7 +191,8 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
     const int q1 = min(ldi(&b.ctr[0]), b.qcap);
     // an empty level ends the forward; so do full lists and a count that left
     // the sigma word (the host reruns the tile either way)
-    const bool stop = (q1 == q0) || ldi(&b.ctr[2]) != 0 || ldi(&b.ctr[1]) != 0;
+    bool stop = ldi(&b.ctr[2]) != 0 || ldi(&b.ctr[1]) != 0;
+    if (!stop && q1 == q0) stop = l >= b.ctr[4] || !seed_pending(b, l);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         b.qoff[l + 1] = q1;
         if (stop) b.ctr[3] = l - 1;
@@ -157,6 +204,15 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
         for (int i = b.qoff[l - 1] + blockIdx.x * BB + threadIdx.x; i < q0; i += gridDim.x * BB) clr[b.Q[i]] = 0;
     }
     const int lane = threadIdx.x & 63;
+    if (b.n_chain > 0 && blockIdx.x == 0 && threadIdx.x < 64) {
+        // chain-source lanes whose endpoints are due at level l+1
+        const int c = b.lane_ch[lane];
+        if (c >= 0) {
+            const int j = b.lane_ch[64 + lane], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
+            if (j == l + 1) push(b, dst, b.ch_L[c], 1ull << lane);
+            if (K - j == l + 1) push(b, dst, b.ch_R[c], 1ull << lane);
+        }
+    }
     for (int base = q0 + gwave() * 64; base < q1; base += nwaves() * 64) {
         const int e = base + lane;
         int j0 = 0, j1 = 0;
@@ -169,14 +225,20 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
         }
         const int d = j1 - j0;
         if (d <= SMALL)
-            for (int j = j0; j < j1; ++j) push(b, dst, b.col[j], F);
+            for (int j = j0; j < j1; ++j) {
+                const int w = b.col[j];
+                if (w >= 0) push(b, dst, w, F);  // (w < 0: a contracted chain)
+            }
         uint64_t bal = __ballot(d > SMALL && !is_hub(b, d));
         while (bal) {
             const int src = __ffsll((long long)bal) - 1;
             bal &= bal - 1;
             const uint64_t FF = shfl64(F, src);
             const int a0 = __shfl(j0, src, 64), a1 = __shfl(j1, src, 64);
-            for (int j = a0 + lane; j < a1; j += 64) push(b, dst, b.col[j], FF);
+            for (int j = a0 + lane; j < a1; j += 64) {
+                const int w = b.col[j];
+                if (w >= 0) push(b, dst, w, FF);
+            }
         }
     }
     // hub slices: the hub's level-l lanes are its mask in mb[l%3]
@@ -185,14 +247,21 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
         const int v = b.hub_v[b.sl_hub[s]];
         const uint64_t F = cur[v];
         if (!F) continue;
-        for (int j = b.sl_j0[s] + lane; j < b.sl_j1[s]; j += 64) push(b, dst, b.col[j], F);
+        for (int j = b.sl_j0[s] + lane; j < b.sl_j1[s]; j += 64) {
+            const int w = b.col[j];
+            if (w >= 0) push(b, dst, w, F);
+        }
     }
 }
 
 // --------------------------------------------------------------- pull ---
+// sigma_w for the lanes N of entry w: the sum over neighbours at level l
+// (fr) in CSR order.  A contracted chain's arc (~c) counts 1 for the lane
+// whose source lies on chain c and is injected at w on this level (sc == c);
+// cm holds the tile's chain-source lanes.
 template <int SM>
 __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, uint64_t N, int a0, int a1,
-                                          uint64_t &acc, double &facc, bool &ovf) {
+                                          uint64_t &acc, double &facc, bool &ovf, int sc, uint64_t cm) {
     const int lane = threadIdx.x & 63;
     const uint64_t me = 1ull << lane;
     for (int c = a0; c < a1; c += 64) {
@@ -201,7 +270,7 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
         uint64_t f = 0;
         if (j < a1) {
             v = b.col[j];
-            f = fr[v] & N;
+            f = v >= 0 ? fr[v] & N : N & cm;
         }
         uint64_t bal = __ballot(f != 0);
         while (bal) {
@@ -217,8 +286,13 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
                     const int vv = __shfl(v, src, 64);
                     const uint64_t ff = shfl64(f, src);
                     if (ff & me) {
-                        use[u] = true;
-                        x[u] = sg<SM>(b, (int64_t)vv * 64 + lane);
+                        if (vv >= 0) {
+                            use[u] = true;
+                            x[u] = sg<SM>(b, (int64_t)vv * 64 + lane);
+                        } else if (~vv == sc) {
+                            use[u] = true;
+                            x[u] = SM == SIG_F64 ? d2u(1.0) : 1ull;
+                        }
                     }
                 }
             }
@@ -268,6 +342,18 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t me = 1ull << lane;
     const bool split = !F64 && b.n_hub > 0;
+    // chain-source lane: its endpoints injected at level l+1 (else -1)
+    int myc = -1, mL = -1, mR = -1;
+    uint64_t cm = 0;
+    if (b.n_chain > 0) {
+        cm = *b.lane_mask;
+        myc = b.lane_ch[lane];
+        if (myc >= 0) {
+            const int j = b.lane_ch[64 + lane], K = b.ch_off[myc + 1] - b.ch_off[myc] + 1;
+            if (j == l + 1) mL = b.ch_L[myc];
+            if (K - j == l + 1) mR = b.ch_R[myc];
+        }
+    }
     int cnt = 0;  // wave-uniform
     for (int base = gwave() * 64; base < b.n; base += nwaves() * 64) {
         const int w0 = base + lane;
@@ -276,13 +362,14 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
         if (!bal) continue;
         const int k0 = cnt;
         // CSR ranges of the chunk's entries, loaded lane-parallel up front
-        int r0 = 0, r1 = 0;
+        int r0 = 0, r1 = 0, e0 = -1;
         if (N0) {
             const int slot = k0 + __popcll(bal & (me - 1));
             s_w[wv][slot] = w0;
             s_m[wv][slot] = N0;
             r0 = b.rp[w0];
             r1 = b.rp[w0 + 1];
+            if (b.n_chain > 0) e0 = b.ep_idx[w0];
         }
         cnt += __popcll(bal);
         __builtin_amdgcn_wave_barrier();
@@ -293,17 +380,19 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
             const int w = __shfl(w0, src, 64);
             const uint64_t N = shfl64(N0, src);
             const int a0 = __shfl(r0, src, 64), a1 = __shfl(r1, src, 64);
+            const int ew = __shfl(e0, src, 64);
             if (!(split && is_hub(b, a1 - a0))) {
                 uint64_t acc = 0;
                 double facc = 0.0;
                 bool ovf = false;
-                sigma_sum<SM>(b, fr, N, a0, a1, acc, facc, ovf);
+                sigma_sum<SM>(b, fr, N, a0, a1, acc, facc, ovf, (w == mL || w == mR) ? myc : -1, cm);
                 if (N & me) {
                     sg_put<SM>(b, (int64_t)w * 64 + lane, F64 ? d2u(facc) : acc);
                     if (!F64 && (ovf || acc > sg_max<SM>() || (b.limit && acc >= b.limit))) b.ctr[1] = 1;
                 }
             }
             if ((N & me) && b.lvl) b.lvl[(int64_t)w * 64 + lane] = l + 1;
+            if ((N & me) && ew >= 0) b.ep_lvl[(int64_t)ew * 64 + lane] = l + 1;
             if (lane == 0) b.vis[w] |= N;
         }
         if (cnt >= STAGE_FLUSH) stage_flush(b, s_w[wv], s_m[wv], cnt, lane);
@@ -322,7 +411,7 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
         double facc = 0.0;
         bool ovf = false;
         constexpr int SI = F64 ? SIG_U64 : SM;  // (not reached in FP64 mode)
-        sigma_sum<SI>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf);
+        sigma_sum<SI>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf, (v == mL || v == mR) ? myc : -1, cm);
         if ((N & me) && acc) {
             const int64_t k = (int64_t)v * 64 + lane;
             uint64_t old, nw;
@@ -356,6 +445,71 @@ struct SumD {
     __device__ __forceinline__ double value() const { return s; }
 };
 
+// ------------------------------------------------- contracted chains ---
+// A chain segment runs from A (level a, path count sA) over K arcs to B
+// (level b, count sB); its interiors y_1 .. y_{K-1} are numbered from A, and
+// PA(i) = sum of r(y)+1 over y_1 .. y_i.  From the two ends alone: y_i is
+// reached from A at level a+i and from B at level b+K-i; with S = b-a+K, the
+// interiors up to mA = ceil(S/2)-1 take A's side (count sA), those past it
+// B's, and for even S the middle one y_{S/2} takes both (count sA + sB, a
+// tie).  Its successors along the chain give dependencies that are prefix
+// sums of r(y)+1 (every path count is constant along a side).
+//
+// Prefix sums of one chain read from a given end: P(x) = sgn * (pre(anchor +
+// sgn x) - pre(anchor)), pre(i) = sum over interiors x_1 .. x_i from L.
+struct ChainPfx {
+    const double *pre;  // the chain's slice of ch_pre (pre(i) at pre[i-1])
+    int anchor, sgn;
+    __device__ __forceinline__ double at(int i) const { return i > 0 ? pre[i - 1] : 0.0; }
+    __device__ __forceinline__ double operator()(int x) const { return sgn * (at(anchor + sgn * x) - at(anchor)); }
+};
+
+// The term the segment adds to rho_A: sum over A's successor y_1 and beyond,
+// PA(mA) / sA, plus the tie interior's share (r+1) / (sA + sB).
+__device__ __forceinline__ double seg_term(int a, int bl, double sA, double sB, int K, const ChainPfx &P) {
+    const int S = bl - a + K;
+    if (S <= 0) return 0.0;
+    int mA;
+    bool tie = false;
+    int t = 0;
+    if ((S & 1) == 0) {
+        t = S >> 1;
+        mA = t - 1;
+        tie = t >= 1 && t <= K - 1;
+    } else {
+        mA = (S - 1) >> 1;
+    }
+    mA = min(mA, K - 1);
+    double term = P(mA) / sA;
+    if (tie) term += (P(t) - P(t - 1)) / (sA + sB);
+    return term;
+}
+
+// The chain terms of endpoint e (vertex at level l, count sv) for this lane:
+// every attached chain is a segment from e to its other endpoint, except the
+// chain holding the lane's own source x_j, which is the segment from e to the
+// source (level 0, count 1, j arcs from L or K - j from R).
+template <int SM>
+__device__ double chain_rho(const BatchDev &b, int e, int l, double sv, int lane) {
+    const int myc = b.lane_ch[lane], myj = b.lane_ch[64 + lane];
+    double rho = 0.0;
+    for (int q = b.ep_off[e]; q < b.ep_off[e + 1]; ++q) {
+        const int code = b.ep_list[q];
+        const int c = code >> 1, side = code & 1;
+        const int off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
+        const ChainPfx P{b.ch_pre + off, side ? K - 1 : 0, side ? -1 : 1};
+        if (c == myc) {
+            rho += seg_term(l, 0, sv, 1.0, side ? K - myj : myj, P);
+        } else {
+            const int o = side ? b.ch_L[c] : b.ch_R[c];
+            const int lv = b.ep_lvl[(int64_t)b.ep_idx[o] * 64 + lane];
+            const double so = sg_val<SM>(sg<SM>(b, (int64_t)o * 64 + lane));
+            rho += seg_term(l, lv, sv, so, K, P);
+        }
+    }
+    return rho;
+}
+
 // rho_v = sum over successors w (lanes in succ[w] & M) of t_w.
 template <typename TT>
 __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *succ, uint64_t M, int a0, int a1) {
@@ -370,7 +524,7 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
         uint64_t x = 0;
         if (j < a1) {
             w = b.col[j];
-            x = succ[w] & M;
+            x = w >= 0 ? succ[w] & M : 0;
         }
         uint64_t bal = __ballot(x != 0);
         while (bal) {
@@ -401,11 +555,13 @@ __device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *suc
 
 // rho complete for (v, lanes M): delta, BC contribution and the term t_v.
 template <int SM, typename TT>
-__device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, uint64_t M, double acc) {
+__device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, int l, uint64_t M, double acc) {
     const int lane = threadIdx.x & 63;
+    const int e = b.n_chain > 0 ? b.ep_idx[v] : -1;
     double contrib = 0.0;
     if (M & (1ull << lane)) {
         const double sv = sg_val<SM>(sg<SM>(b, (int64_t)v * 64 + lane));
+        if (e >= 0) acc += chain_rho<SM>(b, e, l, sv, lane);
         const double delta = sv * acc;
         contrib = b.wsl[lane] * delta;
         ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + acc);
@@ -448,7 +604,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     // are then gathered UNR at a time, accumulated per entry in arc order.
     for (int base = q0 + gwave() * FEB; base < q1; base += nwaves() * FEB) {
         const int e = base + lane;
-        int v0 = 0, r0 = 0, r1 = 0;
+        int v0 = 0, r0 = 0, r1 = 0, ep0 = -1;
         uint64_t M0 = 0;
         if (lane < FEB && e < q1) {
             v0 = b.Q[e];
@@ -456,6 +612,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
             own[v0] = M0;
             r0 = b.rp[v0];
             r1 = b.rp[v0 + 1];
+            if (b.n_chain > 0) ep0 = b.ep_idx[v0];
         }
         const int cnt = min(FEB, q1 - base);
         const bool mine = lane < cnt && !(is_hub(b, r1 - r0) && l < b.hub_stride);  // hubs: slices
@@ -468,7 +625,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         }
         const int pre = inc - d;
         const int T = __shfl(inc, 63, 64);
-        int P[FEB], A[FEB], V[FEB];
+        int P[FEB], A[FEB], V[FEB], EP[FEB];
         uint64_t MM[FEB];
         bool OWN[FEB];
 #pragma unroll
@@ -476,6 +633,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
             P[j] = j < cnt ? __shfl(pre, j, 64) : INT32_MAX;
             A[j] = __shfl(r0, j, 64);
             V[j] = __shfl(v0, j, 64);
+            EP[j] = __shfl(ep0, j, 64);
             MM[j] = shfl64(M0, j);
             OWN[j] = __shfl((int)mine, j, 64) != 0;
             r_rho[wv][j][lane] = 0.0;
@@ -499,7 +657,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                 uint64_t x = 0;
                 if (g < wend) {
                     w = b.col[a0k + g - pk];
-                    x = succ[w] & mk;
+                    x = w >= 0 ? succ[w] & mk : 0;  // (w < 0: a contracted chain)
                 }
                 const uint64_t bal = __ballot(x != 0);
                 if (x) {
@@ -554,7 +712,8 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
             double contrib = 0.0;
             if (MM[j] & me) {
                 const double sv = sg_val<SM>(SX[j]);
-                const double a = r_rho[wv][j][lane];  // rho_v
+                double a = r_rho[wv][j][lane];  // rho_v
+                if (EP[j] >= 0) a += chain_rho<SM>(b, EP[j], l, sv, lane);
                 const double delta = sv * a;
                 contrib = b.wsl[lane] * delta;
                 ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
@@ -581,8 +740,132 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         if (done != b.hub_nsl[h] - 1) continue;
         __threadfence();
         const double rho = __hip_atomic_load(&b.hub_acc[h * 64 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        finish_vertex<SM, TT>(b, v, M, rho);
+        finish_vertex<SM, TT>(b, v, l, M, rho);
         if (lane == 0) b.hub_cnt[h] = 0;
+    }
+}
+
+// Interior y_q (1 <= q <= K-1) of a segment A .. B (see seg_term): its
+// level, path-count word and dependency.  tB is B's term t_B = (r(B)+1 +
+// delta_B) / sigma_B, used when B itself is reached through the segment
+// (b == a + K: only a source's own chain, whose halves end at L and R).
+template <int SM>
+__device__ __forceinline__ void seg_interior(int a, int bl, double sA, double sB, uint64_t wA, uint64_t wB, int K,
+                                             int q, const ChainPfx &P, double tB, int &lev, uint64_t &sw,
+                                             double &dl) {
+    const int S = bl - a + K;
+    int t = -1, mA, b0;
+    bool tie = false;
+    if ((S & 1) == 0) {
+        t = S >> 1;
+        mA = t - 1;
+        tie = t >= 1 && t <= K - 1;
+        b0 = t + 1;
+    } else {
+        mA = (S - 1) >> 1;
+        b0 = mA + 1;
+    }
+    mA = min(mA, K - 1);  // (B beyond the segment's reach: every interior on A's side)
+    const double tw = tie ? P(t) - P(t - 1) : 0.0;
+    if (q <= mA) {
+        lev = a + q;
+        sw = wA;
+        dl = P(mA) - P(q) + (tie ? tw * sA / (sA + sB) : 0.0) + (bl == a + K ? sA * tB : 0.0);
+    } else if (tie && q == t) {
+        lev = a + t;
+        sw = SM == SIG_F64 ? d2u(sA + sB) : wA + wB;
+        dl = 0.0;
+    } else {
+        lev = bl + K - q;
+        sw = wB;
+        dl = P(q - 1) - P(b0 - 1) + (tie ? tw * sB / (sA + sB) : 0.0);
+    }
+}
+
+// Between the forward and the backward: a tie interior's count sA + sB must
+// fit the tile's word (and stay below the test threshold), like every count
+// the pull makes; otherwise the tile is redone with the next word.
+template <int SM>
+__global__ __launch_bounds__(BB) void k_b_chain_check(BatchDev b) {
+    if (ldi(&b.ctr[3]) < 0 || b.ctr[1] || b.ctr[2]) return;
+    const int lane = threadIdx.x & 63;
+    if (lane >= b.ctr[5]) return;
+    const int myc = b.lane_ch[lane], myj = b.lane_ch[64 + lane];
+    for (int c = gwave(); c < b.n_chain; c += nwaves()) {
+        const int L = b.ch_L[c], R = b.ch_R[c], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
+        const int aL = b.ep_lvl[(int64_t)b.ep_idx[L] * 64 + lane];
+        const int aR = b.ep_lvl[(int64_t)b.ep_idx[R] * 64 + lane];
+        const uint64_t wL = sg<SM>(b, (int64_t)L * 64 + lane), wR = sg<SM>(b, (int64_t)R * 64 + lane);
+        // the segments of this lane: L .. R, or source .. L and source .. R;
+        // a tie needs an even S = b - a + K with 2 <= S <= 2K - 2
+        auto tie = [](int S, int k) { return (S & 1) == 0 && S >= 2 && S <= 2 * k - 2; };
+        auto check = [&](uint64_t x, uint64_t y) {
+            const uint64_t z = x + y;
+            if (z < x || z > sg_max<SM>() || (b.limit && z >= b.limit)) b.ctr[1] = 1;
+        };
+        if (c != myc) {
+            if (tie(aR - aL + K, K)) check(wL, wR);
+        } else {
+            if (tie(aL + myj, myj)) check(wL, 1);
+            if (tie(aR + K - myj, K - myj)) check(wR, 1);
+        }
+    }
+}
+
+// After the backward: every contracted interior of every chain, one wave per
+// chain and one lane per source; BC into the slot's accumulator (each
+// interior has one writer), and in the probe the interiors' rows.
+template <int SM, typename TT>
+__global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
+    constexpr bool F64 = SM == SIG_F64;
+    if (ldi(&b.ctr[3]) < 0 || (!F64 && b.ctr[1]) || b.ctr[2]) return;
+    const int lane = threadIdx.x & 63;
+    const bool act = lane < b.ctr[5];
+    const double w = b.wsl[lane];
+    const int myc = b.lane_ch[lane], myj = b.lane_ch[64 + lane];
+    const uint64_t one = F64 ? d2u(1.0) : 1ull;
+    const TT *tt = (const TT *)b.tt;
+    for (int c = gwave(); c < b.n_chain; c += nwaves()) {
+        const int L = b.ch_L[c], R = b.ch_R[c], off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
+        int aL = 0, aR = 0;
+        uint64_t wL = 0, wR = 0;
+        double sL = 1.0, sR = 1.0, tL = 0.0, tR = 0.0;
+        if (act) {
+            aL = b.ep_lvl[(int64_t)b.ep_idx[L] * 64 + lane];
+            aR = b.ep_lvl[(int64_t)b.ep_idx[R] * 64 + lane];
+            wL = sg<SM>(b, (int64_t)L * 64 + lane);
+            wR = sg<SM>(b, (int64_t)R * 64 + lane);
+            sL = sg_val<SM>(wL);
+            sR = sg_val<SM>(wR);
+            if (c == myc) {
+                tL = (double)tt[(int64_t)L * 64 + lane];
+                tR = (double)tt[(int64_t)R * 64 + lane];
+            }
+        }
+        const double *pre = b.ch_pre + off;
+        for (int i = 1; i < K; ++i) {
+            int lev = 0;
+            uint64_t sw = one;
+            double dl = 0.0;
+            if (act) {
+                if (c != myc)
+                    seg_interior<SM>(aL, aR, sL, sR, wL, wR, K, i, ChainPfx{pre, 0, 1}, 0.0, lev, sw, dl);
+                else if (i < myj)  // the half from the source to L, numbered from the source
+                    seg_interior<SM>(0, aL, 1.0, sL, one, wL, myj, myj - i, ChainPfx{pre, myj - 1, -1}, tL, lev, sw,
+                                     dl);
+                else if (i > myj)  // the half from the source to R
+                    seg_interior<SM>(0, aR, 1.0, sR, one, wR, K - myj, i - myj, ChainPfx{pre, myj, 1}, tR, lev, sw,
+                                     dl);
+            }
+            const int64_t x = (int64_t)b.n + off + i - 1;
+            const double contrib = wave_sum(act ? w * dl : 0.0);
+            if (lane == 0 && contrib != 0.0) b.bc[x] += contrib;
+            if (b.lvl && act) {
+                b.lvl[x * 64 + lane] = lev;
+                sg_put<SM>(b, x * 64 + lane, sw);
+                b.dbg_delta[x * 64 + lane] = dl;
+            }
+        }
     }
 }
 
@@ -652,6 +935,21 @@ hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, 
 
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_b_finish, dim3(grid), dim3(BB), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_b_chain_check(const BatchDev &b, int sm, int grid, hipStream_t st) {
+    if (sm == SIG_U32) hipLaunchKernelGGL(k_b_chain_check<SIG_U32>, dim3(grid), dim3(BB), 0, st, b);
+    else if (sm == SIG_U64) hipLaunchKernelGGL(k_b_chain_check<SIG_U64>, dim3(grid), dim3(BB), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_b_chains(const BatchDev &b, int sm, bool fp32, int grid, hipStream_t st) {
+    if (sm == SIG_U32 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U32, float>), dim3(grid), dim3(BB), 0, st, b);
+    else if (sm == SIG_U32) hipLaunchKernelGGL((k_b_chains<SIG_U32, double>), dim3(grid), dim3(BB), 0, st, b);
+    else if (sm == SIG_U64 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U64, float>), dim3(grid), dim3(BB), 0, st, b);
+    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_chains<SIG_U64, double>), dim3(grid), dim3(BB), 0, st, b);
+    else hipLaunchKernelGGL((k_b_chains<SIG_F64, double>), dim3(grid), dim3(BB), 0, st, b);
     return hipGetLastError();
 }
 

@@ -51,6 +51,7 @@ constexpr int GRAPHS_PER_SLOT = 12;
 
 struct Slot {
     DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, hub_mask, hub_acc, hub_cnt, lvl, dbg, bcacc, wsl;
+    DBuf eplv, lanes;  // contracted chains: endpoint levels per lane, lane table
     int32_t n_cap = 0, qcap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -149,7 +150,21 @@ static int bs_ensure(bcx_ctx *ctx, int32_t n, int32_t n_src, int want_slots, boo
     return BCX_OK;
 }
 
-static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uint64_t limit) {
+struct TileJob {
+    const DComp *dc;
+    const int32_t *perm;  // tile vertex id -> component vertex id (nullptr: identity)
+    const ChainCut *cut;  // contracted chains of dc (nullptr: none)
+    double *bc;
+    uint64_t limit;
+    bool fp32;
+    int grid;
+    bool graphs;  // replay captured level launches (first chunk of a tile)
+};
+
+static void set_chains(BatchDev &b, Slot &s, const ChainCut *cut);
+
+static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const TileJob &j, uint64_t limit) {
+    const DComp &dc = *j.dc;
     BatchedState &bs = *ctx->batched;
     BatchDev b{};
     b.n = dc.n;
@@ -185,20 +200,11 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const DComp &dc, double *bc, uin
     b.ws = (const double *)bs.ws.p + (size_t)s.tile * BATCH_W;
     b.wsl = (double *)s.wsl.p;
     b.bc = (double *)s.bcacc.p;
-    (void)bc;
     b.limit = limit;
+    set_chains(b, s, j.cut);
     return b;
 }
 
-struct TileJob {
-    const DComp *dc;
-    const int32_t *perm;  // tile vertex id -> component vertex id (nullptr: identity)
-    double *bc;
-    uint64_t limit;
-    bool fp32;
-    int grid;
-    bool graphs;  // replay captured level launches (first chunk of a tile)
-};
 
 // Forward levels [L, L+C), the speculative backward L+C-1 .. 1 (kernels past
 // the depth or of an unfinished forward return at once), the cleanup, the
@@ -207,8 +213,13 @@ static int launch_levels(bcx_ctx *ctx, Slot &s, const BatchDev &b, int L0, int C
     int rc;
     for (int l = L0; l < L0 + C; ++l)
         if ((rc = ctx_hip(ctx, launch_b_forward(b, s.sm, l, j.grid, s.stream), "forward"))) return rc;
+    if (b.n_chain > 0 && s.sm != SIG_F64 &&
+        (rc = ctx_hip(ctx, launch_b_chain_check(b, s.sm, j.grid, s.stream), "chain check")))
+        return rc;
     for (int l = L0 + C - 1; l >= 1; --l)
         if ((rc = ctx_hip(ctx, launch_b_back(b, s.sm, j.fp32, l, j.grid, s.stream), "back"))) return rc;
+    if (b.n_chain > 0 && (rc = ctx_hip(ctx, launch_b_chains(b, s.sm, j.fp32, j.grid, s.stream), "chains")))
+        return rc;
     return ctx_hip(ctx, launch_b_finish(b, j.grid, s.stream), "finish");
 }
 
@@ -257,7 +268,7 @@ static int replay_levels(bcx_ctx *ctx, Slot &s, BatchDev b, const TileJob &j) {
 // counter read-back and the completion event.  A tile's first chunk replays a
 // captured graph; a deeper tile's later chunks are launched one by one.
 static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st) {
-    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, s.sm == SIG_F64 ? 0 : j.limit);
+    BatchDev b = make_dev(ctx, s, j, s.sm == SIG_F64 ? 0 : j.limit);
     int rc;
     if (j.graphs && s.L == 0) {
         if ((rc = replay_levels(ctx, s, b, j))) return rc;
@@ -266,7 +277,7 @@ static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st)
         return rc;
     }
     s.L += s.C;
-    if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1;
+    if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1 + (b.n_chain > 0 ? 1 + (s.sm != SIG_F64) : 0);
     if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.pinned, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream),
                       "counters")))
         return rc;
@@ -281,7 +292,7 @@ static int start_tile(bcx_ctx *ctx, Slot &s, int tile, int ns, int sm, const Til
     // levels of the first chunk: the last tile's depth + 3, rounded up to a
     // multiple of 8 so that a handful of captured graphs serve every tile
     s.C = (std::max(4, ctx->batched->last_depth + 3) + 7) & ~7;
-    BatchDev b = make_dev(ctx, s, *j.dc, j.bc, sm == SIG_F64 ? 0 : j.limit);
+    BatchDev b = make_dev(ctx, s, j, sm == SIG_F64 ? 0 : j.limit);
     int rc = ctx_hip(ctx, launch_b_init(b, ns, sm, j.grid, s.stream), "init");
     if (rc) return rc;
     if (st) {
@@ -329,7 +340,7 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
     if (s.sm != SIG_F64 && h[1]) {
         // a count left the word's range: the backward did not run; redo the
         // tile with the next wider sigma word
-        BatchDev b = make_dev(ctx, s, *j.dc, j.bc, 0);
+        BatchDev b = make_dev(ctx, s, j, 0);
         if ((rc = ctx_hip(ctx, launch_b_clear_all(b, std::min(h[0], s.qcap), j.grid, s.stream), "clear"))) {
             *rc_out = rc;
             return 0;
@@ -534,6 +545,7 @@ static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double
     TileJob j;
     j.dc = &dc;
     j.perm = nullptr;
+    j.cut = nullptr;
     j.bc = bc;
     j.limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
                   ? (1ull << o->sigma_f64_threshold_log2) : 0;
@@ -843,25 +855,311 @@ static int relabelled(bcx_ctx *ctx, DComp &dc, DComp *&out) {
     return BCX_OK;
 }
 
-int batched_prepare(bcx_ctx *ctx, DComp &dc) {
+// ------------------------------------------------------------------------
+// Contracted chains.  A chain of K arcs between L and R is passive when a
+// path of fewer than K arcs joins L and R outside every chain: no shortest
+// path between two vertices off the chain then runs through it (the path
+// around it is shorter), so its interiors are reached only from its two ends
+// and their levels, counts and dependencies follow from the ends' in closed
+// form.  The bound is min over landmarks h of d(h, L) + d(h, R), with
+// d the BFS distance over the chain-free part of the component from its
+// CUT_LM highest-degree vertices; chains with L == R are kept.
+// ------------------------------------------------------------------------
+constexpr int CUT_LM = 8;
+
+struct bcx::ChainCut {
+    DComp g;                   // the copy: [0, n_core) the tile graph in BFS order, then the interiors
+    int32_t n_core = 0, n_chain = 0, n_ep = 0;
+    std::vector<int32_t> inv;  // component id -> copy id
+    DBuf b_perm;               // copy id -> component id
+    DBuf b_tab;                // the chain tables, one allocation
+    const int32_t *ch_L = nullptr, *ch_R = nullptr, *ch_off = nullptr, *ch_of = nullptr;
+    const int32_t *ep_idx = nullptr, *ep_off = nullptr, *ep_list = nullptr;
+    const double *ch_pre = nullptr;
+};
+
+static void cut_release(bcx_ctx *ctx, DComp &dc) {
+    if (!dc.cut) return;
+    for (DBuf *b : {&dc.cut->g.b_rp, &dc.cut->g.b_col, &dc.cut->g.b_reach, &dc.cut->b_perm, &dc.cut->b_tab})
+        b->release(ctx);
+    delete dc.cut;
+    dc.cut = nullptr;
+}
+
+// Chain ids (into the component's chain tables) of the passive chains.
+static std::vector<int32_t> passive_chains(const HostComp &h, const std::vector<uint8_t> &inner) {
+    const int32_t n = h.n;
+    std::vector<int32_t> lm;
+    for (int32_t v = 0; v < n; ++v)
+        if (!inner[v]) lm.push_back(v);
+    const int K = (int)std::min<size_t>(CUT_LM, lm.size());
+    std::partial_sort(lm.begin(), lm.begin() + K, lm.end(), [&](int32_t a, int32_t b) {
+        const int32_t da = h.rp[a + 1] - h.rp[a], db = h.rp[b + 1] - h.rp[b];
+        return da != db ? da > db : a < b;
+    });
+    std::vector<std::vector<int32_t>> dist((size_t)K);
+    auto bfs = [&](int k) {
+        std::vector<int32_t> &d = dist[(size_t)k];
+        d.assign((size_t)n, -1);
+        std::vector<int32_t> q;
+        q.reserve(n);
+        q.push_back(lm[(size_t)k]);
+        d[lm[(size_t)k]] = 0;
+        for (size_t i = 0; i < q.size(); ++i) {
+            const int32_t v = q[i];
+            for (int32_t j = h.rp[v]; j < h.rp[v + 1]; ++j) {
+                const int32_t w = h.col[j];
+                if (!inner[w] && d[w] < 0) {
+                    d[w] = d[v] + 1;
+                    q.push_back(w);
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 0; k < K; ++k) th.emplace_back(bfs, k);
+    for (auto &x : th) x.join();
+    std::vector<int32_t> out;
+    for (int32_t c = 0; c < h.n_chains; ++c) {
+        const int32_t L = h.ch_ends[2 * c], R = h.ch_ends[2 * c + 1];
+        const int64_t arcs = (int64_t)h.ch_ptr[c + 1] - h.ch_ptr[c] + 1;
+        if (L == R) continue;
+        int64_t ub = INT64_MAX;
+        for (int k = 0; k < K; ++k)
+            if (dist[k][L] >= 0 && dist[k][R] >= 0) ub = std::min<int64_t>(ub, (int64_t)dist[k][L] + dist[k][R]);
+        if (arcs > ub) out.push_back(c);
+    }
+    return out;
+}
+
+// The copy of a component with its passive chains contracted (nullptr when
+// their interiors are fewer than 1% of the vertices: the plain relabelled
+// copy is used then).  Tile vertices keep the BFS order of the relabelled
+// copy; each endpoint row keeps its arcs in the original order with the arc
+// into a contracted chain c encoded as ~c.
+static int chain_cut(bcx_ctx *ctx, DComp &dc, ChainCut *&out) {
+    out = nullptr;
+    if (dc.cut_state == 1) { out = dc.cut; return BCX_OK; }
+    if (dc.cut_state == 0) return BCX_OK;
+    const HostComp &h = dc.host;
+    const int32_t n = h.n;
+    if (h.n_chains == 0) { dc.cut_state = 0; return BCX_OK; }
+    std::vector<uint8_t> inner((size_t)n, 0);
+    for (int32_t x : h.ch_nodes) inner[x] = 1;
+    const std::vector<int32_t> pc = passive_chains(h, inner);
+    int64_t n_in = 0;
+    for (int32_t c : pc) n_in += h.ch_ptr[c + 1] - h.ch_ptr[c];
+    if (n_in == 0 || n_in * 100 < n) { dc.cut_state = 0; return BCX_OK; }
+    const int32_t npc = (int32_t)pc.size();
+    std::vector<int32_t> cidx((size_t)n, -1);  // contracted chain of each interior
+    std::vector<int32_t> off((size_t)npc + 1, 0);
+    for (int32_t k = 0; k < npc; ++k) {
+        const int32_t c = pc[(size_t)k];
+        for (int32_t i = h.ch_ptr[c]; i < h.ch_ptr[c + 1]; ++i) cidx[h.ch_nodes[i]] = k;
+        off[(size_t)k + 1] = off[(size_t)k] + (h.ch_ptr[c + 1] - h.ch_ptr[c]);
+    }
+    if (dc.rank.empty()) dc.rank = bfs_rank(h);
+    std::vector<int32_t> byrank((size_t)n), perm((size_t)n), inv((size_t)n);
+    for (int32_t v = 0; v < n; ++v) byrank[(size_t)dc.rank[v]] = v;
+    int32_t id = 0;
+    for (int32_t r = 0; r < n; ++r) {
+        const int32_t v = byrank[(size_t)r];
+        if (cidx[v] < 0) { inv[v] = id; perm[(size_t)id++] = v; }
+    }
+    const int32_t n_core = id;
+    for (int32_t k = 0; k < npc; ++k) {
+        const int32_t c = pc[(size_t)k];
+        for (int32_t i = h.ch_ptr[c]; i < h.ch_ptr[c + 1]; ++i) {
+            const int32_t x = h.ch_nodes[i];
+            inv[x] = n_core + off[(size_t)k] + (i - h.ch_ptr[c]);
+            perm[(size_t)inv[x]] = x;
+        }
+    }
+    ChainCut *cut = new ChainCut();
+    cut->n_core = n_core;
+    cut->n_chain = npc;
+    HostComp &q = cut->g.host;
+    q.n = n;
+    q.rp.assign((size_t)n + 1, 0);
+    q.reach.resize((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t v = perm[(size_t)i];
+        q.reach[(size_t)i] = h.reach[v];
+        q.rp[(size_t)i + 1] = q.rp[(size_t)i];
+        if (i >= n_core) continue;  // contracted interiors have no arcs
+        for (int32_t j = h.rp[v]; j < h.rp[v + 1]; ++j) {
+            const int32_t w = h.col[j];
+            q.col.push_back(cidx[w] >= 0 ? ~cidx[w] : inv[w]);
+        }
+        q.rp[(size_t)i + 1] = (int32_t)q.col.size();
+    }
+    q.arcs = (int64_t)q.col.size();
+    q.ch_ptr.assign(1, 0);
+    // chain tables
+    std::vector<int32_t> chL((size_t)npc), chR((size_t)npc), chof((size_t)n_in), epidx((size_t)n_core, -1);
+    std::vector<double> pre((size_t)n_in);
+    for (int32_t k = 0; k < npc; ++k) {
+        const int32_t c = pc[(size_t)k];
+        chL[(size_t)k] = inv[h.ch_ends[2 * c]];
+        chR[(size_t)k] = inv[h.ch_ends[2 * c + 1]];
+        double acc = 0.0;
+        for (int32_t i = h.ch_ptr[c]; i < h.ch_ptr[c + 1]; ++i) {
+            const int32_t x = off[(size_t)k] + (i - h.ch_ptr[c]);
+            acc += (double)h.reach[h.ch_nodes[i]] + 1.0;
+            pre[(size_t)x] = acc;
+            chof[(size_t)x] = k;
+        }
+        epidx[chL[(size_t)k]] = 0;
+        epidx[chR[(size_t)k]] = 0;
+    }
+    int32_t n_ep = 0;
+    for (int32_t v = 0; v < n_core; ++v)
+        if (epidx[v] == 0) epidx[v] = ++n_ep;  // (numbered from 1, shifted below)
+    for (int32_t v = 0; v < n_core; ++v)
+        if (epidx[v] > 0) epidx[v]--;
+        else epidx[v] = -1;
+    std::vector<int32_t> epoff((size_t)n_ep + 1, 0), eplist((size_t)2 * npc);
+    for (int32_t k = 0; k < npc; ++k) {
+        epoff[(size_t)epidx[chL[(size_t)k]] + 1]++;
+        epoff[(size_t)epidx[chR[(size_t)k]] + 1]++;
+    }
+    for (int32_t e = 0; e < n_ep; ++e) epoff[(size_t)e + 1] += epoff[(size_t)e];
+    {
+        std::vector<int32_t> fill(epoff.begin(), epoff.end() - 1);
+        for (int32_t k = 0; k < npc; ++k) {
+            eplist[(size_t)fill[(size_t)epidx[chL[(size_t)k]]]++] = 2 * k;
+            eplist[(size_t)fill[(size_t)epidx[chR[(size_t)k]]]++] = 2 * k + 1;
+        }
+    }
+    cut->n_ep = n_ep;
+    // one allocation: the prefix sums (8-byte aligned at the start), then the int tables
+    const size_t ints = 2 * (size_t)npc + ((size_t)npc + 1) + (size_t)n_in + (size_t)n_core + ((size_t)n_ep + 1) +
+                        2 * (size_t)npc;
+    std::vector<unsigned char> tab(8 * (size_t)n_in + 4 * ints);
+    std::memcpy(tab.data(), pre.data(), 8 * (size_t)n_in);
+    size_t at = 8 * (size_t)n_in;
+    std::vector<size_t> pos;
+    for (const std::vector<int32_t> *t : {&chL, &chR, &off, &chof, &epidx, &epoff, &eplist}) {
+        pos.push_back(at);
+        std::memcpy(tab.data() + at, t->data(), 4 * t->size());
+        at += 4 * t->size();
+    }
+    int rc;
+    if ((rc = upload(ctx, q.rp.data(), 4 * q.rp.size(), cut->g.b_rp, "contracted component")) ||
+        (rc = upload(ctx, q.col.data(), 4 * q.col.size(), cut->g.b_col, "contracted component")) ||
+        (rc = upload(ctx, q.reach.data(), 4 * q.reach.size(), cut->g.b_reach, "contracted component")) ||
+        (rc = upload(ctx, perm.data(), 4 * perm.size(), cut->b_perm, "contracted relabelling")) ||
+        (rc = upload(ctx, tab.data(), tab.size(), cut->b_tab, "chain tables")) ||
+        (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "chain tables"))) {
+        dc.cut = cut;
+        cut_release(ctx, dc);
+        return rc;
+    }
+    const unsigned char *base = (const unsigned char *)cut->b_tab.p;
+    cut->ch_pre = (const double *)base;
+    cut->ch_L = (const int32_t *)(base + pos[0]);
+    cut->ch_R = (const int32_t *)(base + pos[1]);
+    cut->ch_off = (const int32_t *)(base + pos[2]);
+    cut->ch_of = (const int32_t *)(base + pos[3]);
+    cut->ep_idx = (const int32_t *)(base + pos[4]);
+    cut->ep_off = (const int32_t *)(base + pos[5]);
+    cut->ep_list = (const int32_t *)(base + pos[6]);
+    DComp &g = cut->g;
+    g.n = n;
+    g.arcs = q.arcs;
+    g.offset = dc.offset;
+    g.dev.n = n;
+    g.dev.arcs = g.arcs;
+    g.dev.rp = (const int32_t *)g.b_rp.p;
+    g.dev.col = (const int32_t *)g.b_col.p;
+    g.dev.reach = (const int32_t *)g.b_reach.p;
+    cut->inv.swap(inv);
+    dc.cut = cut;
+    dc.cut_state = 1;
+    out = cut;
+    return BCX_OK;
+}
+
+static void set_chains(BatchDev &b, Slot &s, const ChainCut *cut) {
+    if (!cut) return;  // (make_dev starts from a zeroed view: n_chain 0)
+    b.n = cut->n_core;
+    b.n_chain = cut->n_chain;
+    b.ch_L = cut->ch_L;
+    b.ch_R = cut->ch_R;
+    b.ch_off = cut->ch_off;
+    b.ch_pre = cut->ch_pre;
+    b.ch_of = cut->ch_of;
+    b.ep_idx = cut->ep_idx;
+    b.ep_off = cut->ep_off;
+    b.ep_list = cut->ep_list;
+    b.ep_lvl = (int32_t *)s.eplv.p;
+    b.lane_ch = (int32_t *)s.lanes.p;
+    b.lane_mask = (uint64_t *)((char *)s.lanes.p + 2 * 64 * sizeof(int32_t));
+}
+
+// Per-slot buffers of a run with contracted chains.
+static int cut_slots(bcx_ctx *ctx, const ChainCut *cut) {
+    BatchedState &bs = *ctx->batched;
+    int rc;
+    for (int i = 0; i < bs.n_slots; ++i) {
+        Slot &s = bs.slots[i];
+        if ((rc = s.eplv.ensure(ctx, 4 * 64 * (size_t)std::max(cut->n_ep, 1), "endpoint levels")) ||
+            (rc = s.lanes.ensure(ctx, 2 * 64 * sizeof(int32_t) + sizeof(uint64_t), "lane chains")))
+            return rc;
+    }
+    return BCX_OK;
+}
+
+static bool want_cut(const bcx_opts *o) { return o->use_chains && !(o->flags & BCX_FLAG_NO_CONTRACT); }
+
+int batched_prepare(bcx_ctx *ctx, DComp &dc, const bcx_opts *o) {
     DComp *pc = nullptr;
+    ChainCut *cut = nullptr;
+    int rc = want_cut(o) ? chain_cut(ctx, dc, cut) : BCX_OK;
+    if (rc || cut) return rc;
     return relabelled(ctx, dc, pc);
+}
+
+// The tile graph of a multi-launch component: the copy with contracted
+// chains when the options ask for it and the component has enough of them,
+// else the relabelled copy.  inv maps component ids to the copy's.
+static int tile_graph(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, DComp *&pc, ChainCut *&cut,
+                      const std::vector<int32_t> *&inv, const int32_t *&perm) {
+    cut = nullptr;
+    int rc = want_cut(o) ? chain_cut(ctx, dc, cut) : BCX_OK;
+    if (rc) return rc;
+    if (cut) {
+        pc = &cut->g;
+        inv = &cut->inv;
+        perm = (const int32_t *)cut->b_perm.p;
+        return BCX_OK;
+    }
+    if ((rc = relabelled(ctx, dc, pc))) return rc;
+    inv = &dc.relabel_inv;
+    perm = (const int32_t *)dc.b_perm.p;
+    return BCX_OK;
 }
 
 int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o, double *bc,
                       bcx_stats *st) {
     DComp *pc = nullptr;
-    int rc = relabelled(ctx, dc, pc);
+    ChainCut *cut = nullptr;
+    const std::vector<int32_t> *inv = nullptr;
+    const int32_t *perm = nullptr;
+    int rc = tile_graph(ctx, dc, o, pc, cut, inv, perm);
     if (rc) return rc;
     std::vector<int32_t> ps(srcs.size());
-    for (size_t i = 0; i < srcs.size(); ++i) ps[i] = dc.relabel_inv[srcs[i]];
+    for (size_t i = 0; i < srcs.size(); ++i) ps[i] = (*inv)[srcs[i]];
     const int64_t ntiles = ((int64_t)ps.size() + BATCH_W - 1) / BATCH_W;
     const int want = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width)) : MAX_SLOTS;
     rc = bs_ensure(ctx, pc->n, (int32_t)(ntiles * BATCH_W), want, false);
     if (rc) return rc;
+    if (cut && (rc = cut_slots(ctx, cut))) return rc;
     if ((rc = stage_sources(ctx, *pc, ps)) || (rc = stage_hubs(ctx, *pc))) return rc;
     TileJob j = make_job(ctx, *pc, o, bc);
-    j.perm = (const int32_t *)dc.b_perm.p;
+    j.perm = perm;
+    j.cut = cut;
+    if (st && cut) st->n_contracted += pc->n - cut->n_core;
     return run_tiles(ctx, *pc, (int64_t)ps.size(), j, st, nullptr);
 }
 
@@ -1204,18 +1502,23 @@ int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *s
         }
     } else if (!rc) {
         DComp *pc = nullptr;
-        rc = relabelled(ctx, dc, pc);
+        ChainCut *cut = nullptr;
+        const std::vector<int32_t> *inv = nullptr;
+        const int32_t *perm = nullptr;
+        rc = tile_graph(ctx, dc, o, pc, cut, inv, perm);
         int sm = SIG_U32;
         if (!rc) {
             std::vector<int32_t> ps((size_t)ns);
-            for (int32_t i = 0; i < ns; ++i) ps[(size_t)i] = dc.relabel_inv[srcs[i]];
-            for (int32_t v = 0; v < n; ++v) at[v] = dc.relabel_inv[v];
+            for (int32_t i = 0; i < ns; ++i) ps[(size_t)i] = (*inv)[srcs[i]];
+            for (int32_t v = 0; v < n; ++v) at[v] = (*inv)[v];
             rc = bs_ensure(ctx, n, BATCH_W, 1, true);
+            if (!rc && cut) rc = cut_slots(ctx, cut);
             if (!rc) rc = stage_sources(ctx, *pc, ps);
             if (!rc) rc = stage_hubs(ctx, *pc);
             if (!rc) {
                 TileJob j = make_job(ctx, *pc, o, bc);
-                j.perm = (const int32_t *)dc.b_perm.p;
+                j.perm = perm;
+                j.cut = cut;
                 rc = run_tiles(ctx, *pc, ns, j, nullptr, &sm);
             }
         }
@@ -1265,7 +1568,7 @@ void batched_release(bcx_ctx *ctx) {
         for (TileGraph &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
         s.graphs.clear();
         for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.qoff, &s.hub_mask, &s.hub_acc,
-                        &s.hub_cnt, &s.lvl, &s.dbg, &s.bcacc, &s.wsl})
+                        &s.hub_cnt, &s.lvl, &s.dbg, &s.bcacc, &s.wsl, &s.eplv, &s.lanes})
             b->release(ctx);
         if (s.pinned) (void)hipHostFree(s.pinned);
         if (s.done) (void)hipEventDestroy(s.done);
@@ -1278,6 +1581,7 @@ void batched_release(bcx_ctx *ctx) {
 
 void batched_graph_release(bcx_ctx *ctx, bcx_dgraph *g) {
     for (DComp &dc : g->comps) {
+        cut_release(ctx, dc);
         if (dc.relabel) {
             for (DBuf *b : {&dc.relabel->b_rp, &dc.relabel->b_col, &dc.relabel->b_reach}) b->release(ctx);
             delete dc.relabel;

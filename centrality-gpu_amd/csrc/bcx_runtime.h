@@ -37,6 +37,8 @@ struct Timer {
 };
 
 // Host copy of a component (planning needs the chains and the degrees).
+struct ChainCut;  // multi-launch copy with contracted chains (bcx_batched_host.hip)
+
 struct HostComp {
     int32_t n = 0;
     int64_t arcs = 0;
@@ -66,6 +68,10 @@ struct DComp {
     DComp *relabel = nullptr;
     DBuf b_perm;
     std::vector<int32_t> relabel_inv;
+    // multi-launch batched path with contracted chains (built on first use;
+    // cut_state -1 not yet tried, 0 the component has too few such chains)
+    ChainCut *cut = nullptr;
+    int8_t cut_state = -1;
 };
 
 // One endpoint-row batch: BFS work (endpoints, with slot and mode) then the
@@ -155,8 +161,9 @@ int batched_persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg);
 void batched_persistent_reset(bcx_ctx *ctx);
 bool batched_narrow(bcx::DComp &dc);
 void small_release(bcx_ctx *ctx);
-// One-time preparation of a multi-launch component (its relabelled copy).
-int batched_prepare(bcx_ctx *ctx, bcx::DComp &dc);
+// One-time preparation of a multi-launch component (its relabelled copy, or
+// the copy with contracted chains when the options ask for chains).
+int batched_prepare(bcx_ctx *ctx, bcx::DComp &dc, const bcx_opts *o);
 int batched_component(bcx_ctx *ctx, bcx::DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o,
                       double *bc, bcx_stats *st);
 // Kernel (BCX_KERNEL_*) the batched path runs a component on, given the

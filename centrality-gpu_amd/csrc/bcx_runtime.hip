@@ -656,7 +656,7 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         if (kind == BCX_KERNEL_WG) {
             wg.push_back({&dc, std::move(tiles)});
         } else {
-            if ((rc = batched_prepare(ctx, dc))) return rc;  // one-time work stays outside the timing
+            if ((rc = batched_prepare(ctx, dc, &o))) return rc;  // one-time work stays outside the timing
             multi.push_back({&dc, std::move(tiles)});
         }
     }

@@ -172,6 +172,15 @@ typedef struct {
  * that size range to the workgroup kernel.                                 */
 #define BCX_FLAG_NO_WG_KERNEL    4
 #define BCX_FLAG_WG_KERNEL       8
+/* Batched path, multi-launch tiles: with use_chains, the chains that no
+ * shortest path between their two endpoints can use (longer than a landmark
+ * bound on the endpoints' distance) are taken out of the tile graph and their
+ * interiors' levels, path counts and dependencies follow in closed form from
+ * the endpoints'; a source inside such a chain runs as a lane seeded at both
+ * endpoints (the reference reduces chain sources through endpoint rows,
+ * Codes/postprocesses.cu:329-360).  NO_CONTRACT keeps every chain vertex in
+ * the tile graph.                                                          */
+#define BCX_FLAG_NO_CONTRACT     16
 
 typedef struct {
     double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */
@@ -189,6 +198,8 @@ typedef struct {
     double  t_chain_s;         /* part of t_kernel_s spent on chain sources   */
     int64_t n_graph_launches;  /* batched: replays of captured launch sequences
                                   (their kernels are counted in n_launches)   */
+    int64_t n_contracted;      /* batched: chain-interior vertices taken out of
+                                  the multi-launch tile graphs (per run)      */
 } bcx_stats;
 
 /* Context: device, streams, scratch.  mem_cap (bytes, 0 = 90% of free
