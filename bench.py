@@ -300,6 +300,14 @@ def main():
     achieved = balg_b / tk_b / 1e9 if tk_b > 0 else None
     traffic = measured_traffic(args.config, src_b / K) if args.scale == 1.0 else None
     launches = sum(st["n_launches"] for st in st_b) / K
+    tiles_run = sum(st["n_tiles"] for st in st_b)
+    tiles = {"flags": args.flags, "contracted_vertices": max(st["n_contracted"] for st in st_b),
+             "components": {"small": st_b[0]["n_comp_small"], "wg": st_b[0]["n_comp_wg"],
+                            "multi": st_b[0]["n_comp_multi"]},
+             "multi_blocks_per_cu": st_b[0]["sched_blocks_per_cu"], "multi_slots": st_b[0]["sched_slots"],
+             "multi_launch_tiles_per_step": tiles_run / K,
+             "list_entries_per_tile": round(sum(st["n_list_entries"] for st in st_b) / tiles_run, 1)
+             if tiles_run else None}
     line = {
         "metric": "exact-BC MTEPS on 1 MI355X: batched FP32-comp vs plain, <=1e-6 rel err",
         "value": round(mteps_b, 3),
@@ -316,6 +324,7 @@ def main():
         "config": {"workload": WORKLOADS.get(args.config, f"workload {args.config}"), "generator": args.config,
                    "n": n, "m": m, "sources": total_src, "slices": args.slices,
                    "slice_sources_timed": src_b, "gpu": arch, "cus": n_cu},
+        "tiles": tiles,
         "plain": plain,
         "speedup_vs_plain": round(mteps_b / plain["mteps"], 3) if plain else None,
         "precision": prec,
@@ -329,7 +338,8 @@ def main():
                      if traffic and tk_b > 0 else None,
                      "launches_per_step": round(launches, 1),
                      "algorithmic_bytes_per_step": round(balg_b / K),
-                     "kernel": "batched kernels of a step (k_b_expand, k_b_pull, k_b_back; k_small; k_wg)"},
+                     "kernel": "batched kernels of a step (k_b_expand, k_b_pull, k_b_back, k_b_chains; "
+                               "k_small; k_wg)"},
         "cpu_baseline": cpu,
     }
     g.free()

@@ -813,8 +813,14 @@ __global__ __launch_bounds__(BB) void k_b_chain_check(BatchDev b) {
 }
 
 // After the backward: every contracted interior of every chain, one wave per
-// chain and one lane per source; BC into the slot's accumulator (each
-// interior has one writer), and in the probe the interiors' rows.
+// chain (the launch has a wave for every chain, so the chains' dependent
+// loads overlap) and one lane per source; BC into the slot's accumulator
+// (each interior has one writer), and in the probe the interiors' rows.  For a lane
+// whose source is off the chain, the segment L .. R's constants (split
+// point, tie, the prefix sums at the split) are set once per chain, so an
+// interior costs the chain's running prefix sum (the same word for every
+// lane), a few selects and the wave sum; the rare lane whose source lies on
+// the chain takes the general form for its two halves.
 template <int SM, typename TT>
 __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
     constexpr bool F64 = SM == SIG_F64;
@@ -827,6 +833,8 @@ __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
     const TT *tt = (const TT *)b.tt;
     for (int c = gwave(); c < b.n_chain; c += nwaves()) {
         const int L = b.ch_L[c], R = b.ch_R[c], off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
+        const double *pre = b.ch_pre + off;
+        const ChainPfx P{pre, 0, 1};
         int aL = 0, aR = 0;
         uint64_t wL = 0, wR = 0;
         double sL = 1.0, sR = 1.0, tL = 0.0, tR = 0.0;
@@ -842,24 +850,65 @@ __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
                 tR = (double)tt[(int64_t)R * 64 + lane];
             }
         }
-        const double *pre = b.ch_pre + off;
+        const bool own = act && c == myc;
+        const bool any_own = __ballot(own) != 0;
+        // segment L .. R seen from L: split, tie and the prefix sums there
+        const int S = aR - aL + K;
+        int t = -1, mA, b0;
+        bool tie = false;
+        if ((S & 1) == 0) {
+            t = S >> 1;
+            mA = t - 1;
+            tie = t >= 1 && t <= K - 1;
+            b0 = t + 1;
+        } else {
+            mA = (S - 1) >> 1;
+            b0 = mA + 1;
+        }
+        mA = min(mA, K - 1);
+        const double tw = tie ? P(t) - P(t - 1) : 0.0;
+        const double cA = P(mA) + (tie ? tw * sL / (sL + sR) : 0.0);
+        const double cB = P(min(b0, K) - 1) - (tie ? tw * sR / (sL + sR) : 0.0);
+        const uint64_t wT = F64 ? d2u(sL + sR) : wL + wR;
+        double p0 = 0.0;  // P(i - 1)
+        // the interiors' BC words, 64 at a time, one per lane (coalesced)
+        double acc = 0.0;
         for (int i = 1; i < K; ++i) {
-            int lev = 0;
-            uint64_t sw = one;
-            double dl = 0.0;
-            if (act) {
-                if (c != myc)
-                    seg_interior<SM>(aL, aR, sL, sR, wL, wR, K, i, ChainPfx{pre, 0, 1}, 0.0, lev, sw, dl);
-                else if (i < myj)  // the half from the source to L, numbered from the source
+            const int64_t x = (int64_t)b.n + off + i - 1;
+            if (((i - 1) & 63) == 0 && i - 1 + lane < K - 1) acc = b.bc[x + lane];
+            const double p1 = pre[i - 1];  // P(i)
+            int lev;
+            uint64_t sw;
+            double dl;
+            if (i <= mA) {
+                lev = aL + i;
+                sw = wL;
+                dl = cA - p1;
+            } else if (tie && i == t) {
+                lev = aL + t;
+                sw = wT;
+                dl = 0.0;
+            } else {
+                lev = aR + K - i;
+                sw = wR;
+                dl = p0 - cB;
+            }
+            if (any_own && own) {
+                lev = 0;
+                sw = one;
+                dl = 0.0;
+                if (i < myj)  // the half from the source to L, numbered from the source
                     seg_interior<SM>(0, aL, 1.0, sL, one, wL, myj, myj - i, ChainPfx{pre, myj - 1, -1}, tL, lev, sw,
                                      dl);
                 else if (i > myj)  // the half from the source to R
                     seg_interior<SM>(0, aR, 1.0, sR, one, wR, K - myj, i - myj, ChainPfx{pre, myj, 1}, tR, lev, sw,
                                      dl);
             }
-            const int64_t x = (int64_t)b.n + off + i - 1;
+            p0 = p1;
             const double contrib = wave_sum(act ? w * dl : 0.0);
-            if (lane == 0 && contrib != 0.0) b.bc[x] += contrib;
+            if (lane == ((i - 1) & 63)) acc += contrib;
+            if ((((i - 1) & 63) == 63 || i == K - 1) && lane <= ((i - 1) & 63))
+                b.bc[(int64_t)b.n + off + ((i - 1) & ~63) + lane] = acc;
             if (b.lvl && act) {
                 b.lvl[x * 64 + lane] = lev;
                 sg_put<SM>(b, x * 64 + lane, sw);
@@ -938,13 +987,22 @@ hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st) {
     return hipGetLastError();
 }
 
+// one wave per chain (2 per block)
+static int chain_grid(const BatchDev &b) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)b.n_chain + NW - 1) / NW, 1 << 16));
+}
+
 hipError_t launch_b_chain_check(const BatchDev &b, int sm, int grid, hipStream_t st) {
+    (void)grid;
+    grid = chain_grid(b);
     if (sm == SIG_U32) hipLaunchKernelGGL(k_b_chain_check<SIG_U32>, dim3(grid), dim3(BB), 0, st, b);
     else if (sm == SIG_U64) hipLaunchKernelGGL(k_b_chain_check<SIG_U64>, dim3(grid), dim3(BB), 0, st, b);
     return hipGetLastError();
 }
 
 hipError_t launch_b_chains(const BatchDev &b, int sm, bool fp32, int grid, hipStream_t st) {
+    (void)grid;
+    grid = chain_grid(b);
     if (sm == SIG_U32 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U32, float>), dim3(grid), dim3(BB), 0, st, b);
     else if (sm == SIG_U32) hipLaunchKernelGGL((k_b_chains<SIG_U32, double>), dim3(grid), dim3(BB), 0, st, b);
     else if (sm == SIG_U64 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U64, float>), dim3(grid), dim3(BB), 0, st, b);

@@ -351,7 +351,11 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
     }
     ctx->batched->last_depth = h[3];
     ctx->batched->last_sm = s.sm;
-    if (st && h[3] > st->max_level) st->max_level = h[3];
+    if (st) {
+        if (h[3] > st->max_level) st->max_level = h[3];
+        st->n_list_entries += h[0];
+        st->n_tiles++;
+    }
     return 0;
 }
 
@@ -541,7 +545,12 @@ bool batched_narrow(DComp &dc) {
     return (int64_t)dc.n <= (int64_t)WG_WIDTH * (dc.depth + 1);
 }
 
-static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double *bc) {
+// level kernels' blocks per CU unless tuned or set (2-wave blocks): 8 measured
+// best on the Chung-Lu and R-MAT cores with 8 tiles in flight (4, 12 and 16
+// within 1-6% below)
+constexpr int DEFAULT_BPC = 8;
+
+static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double *bc, int bpc = DEFAULT_BPC) {
     TileJob j;
     j.dc = &dc;
     j.perm = nullptr;
@@ -550,7 +559,7 @@ static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double
     j.limit = (o->sigma_f64_threshold_log2 > 0 && o->sigma_f64_threshold_log2 < 64)
                   ? (1ull << o->sigma_f64_threshold_log2) : 0;
     j.fp32 = o->precision == BCX_PREC_FP32C;
-    j.grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)dc.n + 3) / 4, (int64_t)ctx->n_cu * 8));
+    j.grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)dc.n + 3) / 4, (int64_t)ctx->n_cu * bpc));
     j.graphs = !(o->flags & BCX_FLAG_NO_GRAPH);
     return j;
 }
@@ -1112,21 +1121,24 @@ static int cut_slots(bcx_ctx *ctx, const ChainCut *cut) {
 
 static bool want_cut(const bcx_opts *o) { return o->use_chains && !(o->flags & BCX_FLAG_NO_CONTRACT); }
 
+static int tile_graph(bcx_ctx *ctx, DComp &dc, bool want, DComp *&pc, ChainCut *&cut,
+                      const std::vector<int32_t> *&inv, const int32_t *&perm);
+
 int batched_prepare(bcx_ctx *ctx, DComp &dc, const bcx_opts *o) {
     DComp *pc = nullptr;
     ChainCut *cut = nullptr;
-    int rc = want_cut(o) ? chain_cut(ctx, dc, cut) : BCX_OK;
-    if (rc || cut) return rc;
-    return relabelled(ctx, dc, pc);
+    const std::vector<int32_t> *inv = nullptr;
+    const int32_t *perm = nullptr;
+    return tile_graph(ctx, dc, want_cut(o) && dc.sched_cut != 0, pc, cut, inv, perm);
 }
 
 // The tile graph of a multi-launch component: the copy with contracted
-// chains when the options ask for it and the component has enough of them,
-// else the relabelled copy.  inv maps component ids to the copy's.
-static int tile_graph(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, DComp *&pc, ChainCut *&cut,
+// chains when asked for and the component has enough of them, else the
+// relabelled copy.  inv maps component ids to the copy's.
+static int tile_graph(bcx_ctx *ctx, DComp &dc, bool want, DComp *&pc, ChainCut *&cut,
                       const std::vector<int32_t> *&inv, const int32_t *&perm) {
     cut = nullptr;
-    int rc = want_cut(o) ? chain_cut(ctx, dc, cut) : BCX_OK;
+    int rc = want ? chain_cut(ctx, dc, cut) : BCX_OK;
     if (rc) return rc;
     if (cut) {
         pc = &cut->g;
@@ -1140,27 +1152,53 @@ static int tile_graph(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, DComp *&pc, Ch
     return BCX_OK;
 }
 
-int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o, double *bc,
-                      bcx_stats *st) {
+// Schedule of a multi-launch component: what the options fix, else what was
+// measured for it, else the defaults.
+struct MultiSched {
+    bool cut;
+    int bpc, slots;
+};
+
+static MultiSched multi_sched(const DComp &dc, const bcx_opts *o) {
+    MultiSched m;
+    m.cut = want_cut(o) && dc.sched_cut != 0;
+    m.bpc = o->blocks_per_cu > 0 ? std::min(o->blocks_per_cu, 64) : (dc.sched_bpc > 0 ? dc.sched_bpc : DEFAULT_BPC);
+    m.slots = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width))
+                                 : (dc.sched_slots > 0 ? dc.sched_slots : MAX_SLOTS);
+    return m;
+}
+
+static int multi_run(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o,
+                     const MultiSched &m, double *bc, bcx_stats *st) {
     DComp *pc = nullptr;
     ChainCut *cut = nullptr;
     const std::vector<int32_t> *inv = nullptr;
     const int32_t *perm = nullptr;
-    int rc = tile_graph(ctx, dc, o, pc, cut, inv, perm);
+    int rc = tile_graph(ctx, dc, m.cut, pc, cut, inv, perm);
     if (rc) return rc;
     std::vector<int32_t> ps(srcs.size());
     for (size_t i = 0; i < srcs.size(); ++i) ps[i] = (*inv)[srcs[i]];
     const int64_t ntiles = ((int64_t)ps.size() + BATCH_W - 1) / BATCH_W;
-    const int want = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width)) : MAX_SLOTS;
-    rc = bs_ensure(ctx, pc->n, (int32_t)(ntiles * BATCH_W), want, false);
+    rc = bs_ensure(ctx, pc->n, (int32_t)(ntiles * BATCH_W), m.slots, false);
     if (rc) return rc;
     if (cut && (rc = cut_slots(ctx, cut))) return rc;
     if ((rc = stage_sources(ctx, *pc, ps)) || (rc = stage_hubs(ctx, *pc))) return rc;
-    TileJob j = make_job(ctx, *pc, o, bc);
+    TileJob j = make_job(ctx, *pc, o, bc, m.bpc);
     j.perm = perm;
     j.cut = cut;
     if (st && cut) st->n_contracted += pc->n - cut->n_core;
     return run_tiles(ctx, *pc, (int64_t)ps.size(), j, st, nullptr);
+}
+
+int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, const bcx_opts *o, double *bc,
+                      bcx_stats *st) {
+    return multi_run(ctx, dc, srcs, o, multi_sched(dc, o), bc, st);
+}
+
+void batched_sched_report(const DComp &dc, const bcx_opts *o, int32_t *bpc, int32_t *slots) {
+    const MultiSched m = multi_sched(dc, o);
+    *bpc = m.bpc;
+    *slots = m.slots;
 }
 
 // ------------------------------------------------------------------------
@@ -1450,6 +1488,122 @@ int batched_kind(DComp &dc, int flags) {
     return BCX_KERNEL_MULTI;
 }
 
+// ------------------------------------------------------------------------
+// Schedule tuner.  A component of at least TUNE_N vertices is run once on
+// sample tiles (evenly spaced over its tile order, BC into a scratch
+// accumulator, timed with HIP events, outside the run's timing) under each
+// candidate schedule, and the fastest is cached for every later run:
+//  * kernel, for components the workgroup kernel can take (<= WG_N vertices):
+//    the workgroup kernel over min(tiles, CUs) sample tiles (its concurrent
+//    width) against the multi-launch tiles over TUNE_TILES;
+//  * multi-launch: 8 / 4 / 16 level-kernel blocks per CU, 8 / 4 tiles in
+//    flight, chain contraction on / off.
+// Each candidate runs twice (the first run captures its graphs) and is timed
+// on the second; a candidate replaces the default only when its rate
+// (sources per second) is at least 3% higher.  Options that fix a choice are
+// kept.  (The reference picks its schedule from sampled BFS depths and
+// frontier sizes, Codes/kernels_free.cu:447-470, 288, 381.)
+// ------------------------------------------------------------------------
+constexpr int32_t TUNE_N = 16384;
+constexpr int TUNE_TILES = 24;
+
+// Sources of k tiles evenly spaced over the component's tile order.
+static std::vector<int32_t> sample_tiles(DComp &dc, int64_t k) {
+    const std::vector<int32_t> order = batched_order(dc);
+    const int64_t T = ((int64_t)dc.n + BATCH_W - 1) / BATCH_W;
+    k = std::min(k, T);
+    std::vector<int32_t> out;
+    for (int64_t i = 0; i < k; ++i) {
+        const int64_t t = (2 * i + 1) * T / (2 * k);
+        for (int64_t v = t * BATCH_W; v < std::min<int64_t>((t + 1) * BATCH_W, dc.n); ++v)
+            out.push_back(order[(size_t)v]);
+    }
+    return out;
+}
+
+int batched_schedule(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, bcx_stats *st) {
+    const int kind0 = batched_kind(dc, o->flags);
+    if (kind0 == BCX_KERNEL_SMALL) return kind0;
+    if ((o->flags & BCX_FLAG_NO_TUNE) || dc.n < TUNE_N) return kind0;
+    const bool kind_fixed = (o->flags & (BCX_FLAG_NO_WG_KERNEL | BCX_FLAG_WG_KERNEL)) || dc.n > WG_N;
+    int kind = kind_fixed ? kind0 : dc.sched_kind;
+    const bool need_kind = kind < 0;
+    const bool need_multi = !dc.sched_multi && (need_kind || kind == BCX_KERNEL_MULTI);
+    if (!need_kind && !need_multi) return kind;
+    // the run's timer is set aside: the samples are timed on their own
+    Timer keep;
+    std::swap(keep, ctx->timer);
+    DBuf scratch;
+    int rc = scratch.ensure(ctx, 8 * ((size_t)dc.offset + (size_t)dc.n), "tuner scratch");
+    bcx_opts to = *o;
+    to.src_subset = nullptr;
+    to.n_src = 0;
+    double *bc = (double *)scratch.p;
+    auto time_multi = [&](const std::vector<int32_t> &srcs, const MultiSched &m, double *rate) {
+        double t = 0.0;
+        int r = BCX_OK;
+        for (int rep = 0; rep < 2 && !r; ++rep) {
+            r = multi_run(ctx, dc, srcs, &to, m, bc + dc.offset, nullptr);
+            t = ctx->timer.collect();
+        }
+        *rate = t > 0 ? (double)srcs.size() / t : 0.0;
+        return r;
+    };
+    auto time_wg = [&](const std::vector<int32_t> &srcs, double *rate) {
+        double t = 0.0;
+        int r = BCX_OK;
+        std::vector<SmallJob> jobs{{&dc, srcs}};
+        for (int rep = 0; rep < 2 && !r; ++rep) {
+            r = persistent_start(ctx, jobs, &to, bc, nullptr, true, ctx->stream, false);
+            if (!r) r = persistent_finish(ctx, nullptr, true);
+            t = ctx->timer.collect();
+        }
+        *rate = t > 0 ? (double)srcs.size() / t : 0.0;
+        return r;
+    };
+    const MultiSched base = multi_sched(dc, o);  // the defaults, or what the options fix
+    const std::vector<int32_t> samp = rc ? std::vector<int32_t>() : sample_tiles(dc, TUNE_TILES);
+    double r_multi = 0.0;
+    if (!rc) rc = time_multi(samp, base, &r_multi);
+    if (!rc && need_kind) {
+        double r_wg = 0.0;
+        rc = time_wg(sample_tiles(dc, std::max(1, ctx->n_cu)), &r_wg);
+        kind = r_wg >= r_multi ? BCX_KERNEL_WG : BCX_KERNEL_MULTI;
+    }
+    MultiSched best = base;
+    const bool tune_multi = need_multi && kind == BCX_KERNEL_MULTI;
+    if (!rc && tune_multi) {
+        std::vector<MultiSched> cands;
+        if (o->blocks_per_cu <= 0)
+            for (int bpc : {4, 16}) cands.push_back({base.cut, bpc, base.slots});
+        if (o->batch_width <= 0) cands.push_back({base.cut, base.bpc, 4});
+        if (base.cut) cands.push_back({false, base.bpc, base.slots});
+        double r_best = r_multi;
+        for (const MultiSched &m : cands) {
+            double r = 0.0;
+            if ((rc = time_multi(samp, m, &r))) break;
+            if (r > 1.03 * r_best) {
+                r_best = r;
+                best = m;
+            }
+        }
+    }
+    std::swap(keep, ctx->timer);
+    keep.destroy();
+    scratch.release(ctx);
+    if (rc) return rc;
+    if (need_kind) dc.sched_kind = (int8_t)kind;
+    if (tune_multi) {
+        // (a choice the options fixed is not cached: later runs may leave it free)
+        dc.sched_multi = true;
+        dc.sched_cut = (base.cut && !best.cut) ? 0 : 1;
+        dc.sched_bpc = (int16_t)(o->blocks_per_cu > 0 ? 0 : best.bpc);
+        dc.sched_slots = (int16_t)(o->batch_width > 0 ? 0 : best.slots);
+    }
+    if (st) st->n_tuned++;
+    return kind;
+}
+
 // One tile of srcs[0..ns) on the kernel bcx_run would pick for the component
 // (the multi-launch tiles on the component's BFS-ordered copy, as in a run),
 // with the probe arrays of that kernel switched on; every lane is read back.
@@ -1505,7 +1659,7 @@ int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *s
         ChainCut *cut = nullptr;
         const std::vector<int32_t> *inv = nullptr;
         const int32_t *perm = nullptr;
-        rc = tile_graph(ctx, dc, o, pc, cut, inv, perm);
+        rc = tile_graph(ctx, dc, want_cut(o), pc, cut, inv, perm);
         int sm = SIG_U32;
         if (!rc) {
             std::vector<int32_t> ps((size_t)ns);

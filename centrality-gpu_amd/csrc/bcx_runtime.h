@@ -72,6 +72,14 @@ struct DComp {
     // cut_state -1 not yet tried, 0 the component has too few such chains)
     ChainCut *cut = nullptr;
     int8_t cut_state = -1;
+    // batched path: the schedule measured on sample tiles, cached for every
+    // later run of the component (sched_kind -1: kernel not measured;
+    // sched_multi: the multi-launch fields below were measured)
+    int8_t sched_kind = -1;
+    bool sched_multi = false;
+    int8_t sched_cut = 1;     // multi-launch: 1 contracted chains, 0 the relabelled copy
+    int16_t sched_bpc = 0;    // multi-launch: level-kernel blocks per CU (0: default)
+    int16_t sched_slots = 0;  // multi-launch: tiles in flight (0: default)
 };
 
 // One endpoint-row batch: BFS work (endpoints, with slot and mode) then the
@@ -169,6 +177,13 @@ int batched_component(bcx_ctx *ctx, bcx::DComp &dc, const std::vector<int32_t> &
 // Kernel (BCX_KERNEL_*) the batched path runs a component on, given the
 // bcx_opts flags.
 int batched_kind(bcx::DComp &dc, int flags);
+// The kernel and, for the multi-launch tiles, the schedule of a component of
+// the batched path: measured once on sample tiles for components of at least
+// TUNE_N vertices (unless the options fix it), else batched_kind's rule.
+// Returns a BCX_KERNEL_* or a negative BCX_E_* status.
+int batched_schedule(bcx_ctx *ctx, bcx::DComp &dc, const bcx_opts *o, bcx_stats *st);
+// Blocks per CU and tiles in flight a multi-launch run of dc uses.
+void batched_sched_report(const bcx::DComp &dc, const bcx_opts *o, int32_t *bpc, int32_t *slots);
 int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns, const bcx_opts *o,
                        int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta, int32_t *sigma_is_f64,
                        int32_t *kernel_used);

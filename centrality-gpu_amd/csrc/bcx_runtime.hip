@@ -618,6 +618,7 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
     std::vector<SmallJob> small;  // batched path: components for the one-launch small kernel
     std::vector<SmallJob> wg;     // batched path: deep components for the workgroup-per-tile kernel
     std::vector<SmallJob> multi;  // batched path: components for the multi-launch tiles
+    int32_t largest_multi = -1;
     for (size_t ci = 0; ci < g->comps.size(); ++ci) {
         DComp &dc = g->comps[ci];
         const int32_t *sp = o.src_subset ? sub[ci].data() : nullptr;
@@ -645,8 +646,10 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         int64_t n_chain = 0;
         std::vector<int32_t> srcs = batched_sources(dc, sp, sn, o.use_chains ? &n_chain : nullptr);
         account((int64_t)srcs.size() - n_chain, n_chain);
-        const int kind = batched_kind(dc, o.flags);
+        const int kind = batched_schedule(ctx, dc, &o, s);  // (one-time sample runs stay outside the timing)
+        if (kind < 0) return kind;
         if (kind == BCX_KERNEL_SMALL) {
+            s->n_comp_small++;
             small.push_back({&dc, std::move(srcs)});
             continue;
         }
@@ -654,9 +657,15 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
         std::vector<int32_t> tiles = tuned_sources(dc, srcs);
         if (dc.tile_order == 1) s->n_ball_tiles++;
         if (kind == BCX_KERNEL_WG) {
+            s->n_comp_wg++;
             wg.push_back({&dc, std::move(tiles)});
         } else {
             if ((rc = batched_prepare(ctx, dc, &o))) return rc;  // one-time work stays outside the timing
+            s->n_comp_multi++;
+            if (dc.n > largest_multi) {
+                largest_multi = dc.n;
+                batched_sched_report(dc, &o, &s->sched_blocks_per_cu, &s->sched_slots);
+            }
             multi.push_back({&dc, std::move(tiles)});
         }
     }

@@ -155,6 +155,8 @@ typedef struct {
     int32_t        sigma_f64_threshold_log2; /* test hook: treat sigma >=
                                      2^k as overflow (0 = 64, i.e. uint64)  */
     int32_t        flags;         /* BCX_FLAG_* bits                          */
+    int32_t        blocks_per_cu; /* batched multi-launch tiles: level-kernel
+                                     blocks per compute unit (0 = tuned)     */
 } bcx_opts;
 
 /* Batched path: run components of up to 2048 vertices through the
@@ -181,6 +183,16 @@ typedef struct {
  * Codes/postprocesses.cu:329-360).  NO_CONTRACT keeps every chain vertex in
  * the tile graph.                                                          */
 #define BCX_FLAG_NO_CONTRACT     16
+/* Batched path: components of at least 16384 vertices get a schedule
+ * measured once on sample tiles and cached (no counterpart in the
+ * reference, which switches schedule from sampled BFS depths,
+ * Codes/kernels_free.cu:447-470): the workgroup or multi-launch kernel for
+ * components of up to 131072 vertices, and for multi-launch components the
+ * level kernels' blocks per CU, the tiles in flight and chain contraction
+ * on or off.  NO_TUNE keeps the defaults (the level-width rule, 8 blocks
+ * per CU, 8 tiles, contraction on).  Options that fix a choice (the kernel
+ * flags, batch_width, blocks_per_cu, NO_CONTRACT) are kept and not tuned.  */
+#define BCX_FLAG_NO_TUNE         32
 
 typedef struct {
     double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */
@@ -200,6 +212,16 @@ typedef struct {
                                   (their kernels are counted in n_launches)   */
     int64_t n_contracted;      /* batched: chain-interior vertices taken out of
                                   the multi-launch tile graphs (per run)      */
+    int64_t n_list_entries;    /* batched, multi-launch tiles: (vertex, level)
+                                  list entries of the completed tiles         */
+    int64_t n_tiles;           /* batched, multi-launch tiles completed       */
+    int32_t n_comp_small;      /* batched: components run on the small-component */
+    int32_t n_comp_wg;         /*   kernel, the workgroup kernel and the      */
+    int32_t n_comp_multi;      /*   multi-launch tiles                        */
+    int32_t n_tuned;           /* batched: components whose schedule was measured
+                                  (the schedule of each is measured once)     */
+    int32_t sched_blocks_per_cu; /* batched: blocks per CU and tiles in flight */
+    int32_t sched_slots;       /*   of the largest multi-launch component     */
 } bcx_stats;
 
 /* Context: device, streams, scratch.  mem_cap (bytes, 0 = 90% of free
