@@ -128,7 +128,8 @@ def summarize_trace(path):
               f"span first start to last end {(max(e for _, e in b) - min(s for s, _ in b)) / 1e6:.1f} ms")
 
 
-TRAFFIC_FILES = {2: "batched_config2_traffic.json", 4: "wg_config4_traffic.json"}
+TRAFFIC_FILES = {2: "batched_config2_traffic.json", 3: "batched_config3_traffic.json",
+                 4: "wg_config4_traffic.json", 5: "batched_config5_traffic.json"}
 WORKLOADS = {
     1: "16 Erdos-Renyi blocks of 128 vertices on a tree of cut vertices, 20% of edges as chains, "
        "512 pendant vertices (10k vertices)",
