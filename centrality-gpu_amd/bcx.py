@@ -315,6 +315,7 @@ FLAG_NO_WG_KERNEL = 4
 FLAG_WG_KERNEL = 8
 FLAG_NO_CONTRACT = 16
 FLAG_NO_TUNE = 32
+FLAG_TILE64 = 64
 KERNEL_SMALL, KERNEL_WG, KERNEL_MULTI = 1, 2, 3
 
 

@@ -1,32 +1,37 @@
 // Batched multi-source Brandes for gfx950: one tile = one component and up
-// to 64 sources, one source per lane of a wave64.  State is vertex-major so a
-// single gather serves every source of the tile:
-//   vis[v]        u64  lanes that have reached v
-//   mb[l % 3][v]  u64  lanes for which v is at level l (three rotating dense
-//                      mask arrays; only entries of the level lists are ever
-//                      non-zero, cleared incrementally)
-//   sig[v][64]    u32 or u64 (exact) or f64 (overflow mode) path counts per
-//                 lane; the narrowest word that holds the tile's counts
-//   tt[v][64]     f32 or f64 backward term t_v = (r(v)+1)/sigma_v + rho_v
-//   Q / Qm        per-level lists of (vertex, lane mask), levels concatenated,
-//                 level l = Q[qoff[l], qoff[l+1])
+// to 64 W sources (W = 1 or 2 per lane of a wave64: lane k owns sources
+// k + 64 h).  State is vertex-major so a single gather serves every source of
+// the tile:
+//   vis[v]        W u64  sources that have reached v
+//   mb[l % 3][v]  W u64  sources for which v is at level l (three rotating
+//                        dense mask arrays; only entries of the level lists are
+//                        ever non-zero, cleared incrementally)
+//   sig[v][64][W] u32 or u64 (exact) or f64 (overflow mode) path counts per
+//                 source; the narrowest word that holds the tile's counts
+//   tt[v][64][W]  f32 or f64 backward term t_v = (r(v)+1)/sigma_v + rho_v
+//   Q / Qm        per-level lists of (vertex, source mask), levels
+//                 concatenated, level l = Q[qoff[l], qoff[l+1])
+// With W = 2 a (vertex, level) entry, its arc walk and its mask gathers serve
+// 128 sources: the tile reaches a vertex at few more distinct levels than a
+// 64-source tile does, so the entries and arcs walked per source fall by
+// close to half, while a lane's two row words stay one 8- or 16-byte load.
 //
 // Forward level l, two launches:
-//   expand  pushes lane masks from Q_l to neighbours, F & ~vis[w], with
+//   expand  pushes source masks from Q_l to neighbours, F & ~vis[w], with
 //           fire-and-forget 64-bit atomic ORs into mb[(l+1)%3] (a lane walks
 //           the arcs of a low-degree entry, a wave those of a larger one);
 //   pull    scans mb[(l+1)%3] in coalesced 64-vertex chunks; every non-zero
 //           entry w joins Q_{l+1} and gets sigma_w = sum of sigma_v over
-//           neighbours v in CSR order for the lanes where v is at level l --
-//           no sigma atomics, and the FP64 order is the oracle's, so FP64
+//           neighbours v in CSR order for the sources where v is at level l
+//           -- no sigma atomics, and the FP64 order is the oracle's, so FP64
 //           sigma is bit-exact.  (Codes/kernels_free.cu:286-362 computes the
 //           same quantities one source per block with atomics.)
 // Backward level l: rho_v = sum over successors w of t_w, with
 // rho = delta / sigma, so the successor-method term
 // sigma_v/sigma_w (r(w)+1 + delta_w) (Codes/kernels_free.cu:378-421) becomes a
-// plain sum of one gathered word per (arc, lane); FP32 with Kahan
+// plain sum of one gathered word per (arc, source); FP32 with Kahan
 // compensation (FP64 in FP64 precision and in FP64-sigma tiles).
-// BC(v) += sum_lanes (r(s)+1) sigma_v rho_v (Codes/kernels_free.cu:423-426),
+// BC(v) += sum_sources (r(s)+1) sigma_v rho_v (Codes/kernels_free.cu:423-426),
 // into the slot's own FP64 accumulator (a vertex is finished once per level
 // launch, so no atomics), folded into the component at the end of a run.
 //
@@ -71,6 +76,92 @@ __device__ __forceinline__ int ldi(const int32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ------------------------------------------------------- source masks ---
+template <int W>
+struct Msk {
+    uint64_t m[W];
+};
+template <int W>
+__device__ __forceinline__ Msk<W> mzero() {
+    Msk<W> r;
+#pragma unroll
+    for (int h = 0; h < W; ++h) r.m[h] = 0;
+    return r;
+}
+template <int W>
+__device__ __forceinline__ bool many(const Msk<W> &a) {
+    uint64_t x = 0;
+#pragma unroll
+    for (int h = 0; h < W; ++h) x |= a.m[h];
+    return x != 0;
+}
+template <int W>
+__device__ __forceinline__ Msk<W> mand(const Msk<W> &a, const Msk<W> &c) {
+    Msk<W> r;
+#pragma unroll
+    for (int h = 0; h < W; ++h) r.m[h] = a.m[h] & c.m[h];
+    return r;
+}
+template <int W>
+__device__ __forceinline__ Msk<W> mandn(const Msk<W> &a, const Msk<W> &c) {  // a & ~c
+    Msk<W> r;
+#pragma unroll
+    for (int h = 0; h < W; ++h) r.m[h] = a.m[h] & ~c.m[h];
+    return r;
+}
+// the W bits of lane k: bit h = source k + 64 h
+template <int W>
+__device__ __forceinline__ int mbits(const Msk<W> &a, int k) {
+    int r = 0;
+#pragma unroll
+    for (int h = 0; h < W; ++h) r |= (int)((a.m[h] >> k) & 1) << h;
+    return r;
+}
+template <int W>
+__device__ __forceinline__ Msk<W> mshfl(const Msk<W> &a, int src) {
+    Msk<W> r;
+#pragma unroll
+    for (int h = 0; h < W; ++h) r.m[h] = shfl64(a.m[h], src);
+    return r;
+}
+template <int W>
+__device__ __forceinline__ Msk<W> mld(const uint64_t *p, int64_t i) {
+    Msk<W> r;
+    if constexpr (W == 2) {
+        const ulonglong2 y = ((const ulonglong2 *)p)[i];
+        r.m[0] = y.x;
+        r.m[1] = y.y;
+    } else {
+        r.m[0] = p[i];
+    }
+    return r;
+}
+template <int W>
+__device__ __forceinline__ void mst(uint64_t *p, int64_t i, const Msk<W> &a) {
+    if constexpr (W == 2) {
+        ulonglong2 y;
+        y.x = a.m[0];
+        y.y = a.m[1];
+        ((ulonglong2 *)p)[i] = y;
+    } else {
+        p[i] = a.m[0];
+    }
+}
+// fire-and-forget OR of the non-zero words (results unused: no return)
+template <int W>
+__device__ __forceinline__ void mor(uint64_t *p, int64_t i, const Msk<W> &a) {
+#pragma unroll
+    for (int h = 0; h < W; ++h)
+        if (a.m[h]) atomicOr((unsigned long long *)&p[i * W + h], (unsigned long long)a.m[h]);
+}
+template <int W>
+__device__ __forceinline__ Msk<W> mbit(int h, int k) {
+    Msk<W> r = mzero<W>();
+    r.m[h] = 1ull << k;
+    return r;
+}
+
+// ------------------------------------------------------------ rows ---
 // sigma word i of the tile in mode SM, widened to 64 bits (f64 as its bits)
 template <int SM>
 __device__ __forceinline__ uint64_t sg(const BatchDev &b, int64_t i) {
@@ -91,16 +182,46 @@ template <int SM>
 __device__ __forceinline__ uint64_t sg_max() {
     return SM == SIG_U32 ? 0xffffffffull : ~0ull;
 }
+// the W sigma words of lane cell c = v * 64 + k, one load
+template <int SM, int W>
+__device__ __forceinline__ void sg_row(const BatchDev &b, int64_t c, uint64_t (&x)[W]) {
+    if constexpr (W == 2) {
+        if constexpr (SM == SIG_U32) {
+            const uint2 y = ((const uint2 *)b.sig)[c];
+            x[0] = y.x;
+            x[1] = y.y;
+        } else {
+            const ulonglong2 y = ((const ulonglong2 *)b.sig)[c];
+            x[0] = y.x;
+            x[1] = y.y;
+        }
+    } else {
+        x[0] = sg<SM>(b, c);
+    }
+}
+template <typename TT, int W>
+__device__ __forceinline__ void tt_row(const BatchDev &b, int64_t c, TT (&x)[W]) {
+    if constexpr (W == 2) {
+        using V2 = typename std::conditional<sizeof(TT) == 4, float2, double2>::type;
+        const V2 y = ((const V2 *)b.tt)[c];
+        x[0] = y.x;
+        x[1] = y.y;
+    } else {
+        x[0] = ((const TT *)b.tt)[c];
+    }
+}
 
 // ---------------------------------------------------------------- init ---
+template <int W>
 __global__ __launch_bounds__(BB) void k_b_clear_vis(BatchDev b, int sm) {
     const int64_t stride = (int64_t)gridDim.x * BB;
     const int64_t t0 = blockIdx.x * (int64_t)BB + threadIdx.x;
-    for (int64_t v = t0; v < b.n; v += stride) b.vis[v] = 0;
+    for (int64_t v = t0; v < (int64_t)b.n * W; v += stride) b.vis[v] = 0;
     // hub sigma rows and backward accumulators start at zero every tile: each
-    // (hub, lane) is summed into exactly once
-    for (int64_t i = t0; i < (int64_t)b.n_hub * 64; i += stride) {
-        const int64_t k = (int64_t)b.hub_v[i >> 6] * 64 + (i & 63);
+    // (hub, source) is summed into exactly once
+    constexpr int64_t ROW = 64 * W;
+    for (int64_t i = t0; i < (int64_t)b.n_hub * ROW; i += stride) {
+        const int64_t k = (int64_t)b.hub_v[i / ROW] * ROW + (i % ROW);
         if (sm == SIG_U32) sg_put<SIG_U32>(b, k, 0);
         else sg_put<SIG_U64>(b, k, 0);
         b.hub_acc[i] = 0.0;
@@ -108,83 +229,100 @@ __global__ __launch_bounds__(BB) void k_b_clear_vis(BatchDev b, int sm) {
     for (int64_t i = t0; i < b.n_hub; i += stride) b.hub_cnt[i] = 0;
 }
 
-// Lanes of a tile vertex start at level 0.  A lane whose source is the
+// Sources that are tile vertices start at level 0.  A source that is the
 // interior x_j of a contracted chain (ids >= n) has no vertex at level 0: the
 // expand kernel injects it at L and at R, each with a path count of 1 (its
 // own path along the chain) on levels j and K - j, and the forward runs on
-// through empty levels while an injection into a vertex the lane has not yet
-// reached is pending.  (Shifting such a lane's levels down so that its nearer
-// endpoint enters at level 1 cuts a tile's depth but misaligns it with the
-// tile's other lanes, which the tile order groups by their true distances:
-// 115k against 123k MTEPS on the 1M-vertex mixed graph.)
-template <int SM>
+// through empty levels while an injection into a vertex the source has not
+// yet reached is pending.  (Shifting such a source's levels down so that its
+// nearer endpoint enters at level 1 cuts a tile's depth but misaligns it with
+// the tile's other sources, which the tile order groups by their true
+// distances: 115k against 123k MTEPS on the 1M-vertex mixed graph.)
+template <int SM, int W>
 __global__ void k_b_seed(BatchDev b, int ns) {
     const int k = threadIdx.x;  // (launched with 64 threads)
-    b.wsl[k] = k < ns ? b.ws[k] : 0.0;
-    int s = k < ns ? b.srcs[k] : -1;
-    int c = -1, j = 0, top = 0;
-    if (s >= b.n) {
-        const int x = s - b.n;
-        c = b.ch_of[x];
-        j = x - b.ch_off[c] + 1;
-        top = max(j, b.ch_off[c + 1] - b.ch_off[c] + 1 - j);
-        s = -1;
+    int top = 0, base = 0;
+    uint64_t chl[W];
+#pragma unroll
+    for (int h = 0; h < W; ++h) {
+        const int i = h * 64 + k;
+        b.wsl[i] = i < ns ? b.ws[i] : 0.0;
+        int s = i < ns ? b.srcs[i] : -1;
+        int c = -1, j = 0;
+        if (s >= b.n) {
+            const int x = s - b.n;
+            c = b.ch_of[x];
+            j = x - b.ch_off[c] + 1;
+            top = max(top, max(j, b.ch_off[c + 1] - b.ch_off[c] + 1 - j));
+            s = -1;
+        }
+        if (b.n_chain > 0) {
+            b.lane_ch[i] = c;
+            b.lane_ch[64 * W + i] = j;
+        }
+        const uint64_t core = __ballot(s >= 0);
+        chl[h] = __ballot(c >= 0);
+        if (s >= 0) {
+            const uint64_t bit = 1ull << k;
+            const int pos = base + __popcll(core & (bit - 1));
+            const Msk<W> m = mbit<W>(h, k);
+            const int64_t cell = ((int64_t)s * 64 + k) * W + h;
+            mst<W>(b.vis, s, m);
+            mst<W>(b.mb0, s, m);
+            sg_put<SM>(b, cell, SM == SIG_F64 ? d2u(1.0) : 1ull);
+            b.Q[pos] = s;
+            mst<W>(b.Qm, pos, m);
+            if (b.lvl) b.lvl[cell] = 0;
+            if (b.n_chain > 0 && b.ep_idx[s] >= 0) b.ep_lvl[((int64_t)b.ep_idx[s] * 64 + k) * W + h] = 0;
+        }
+        base += __popcll(core);
     }
-    if (b.n_chain > 0) {
-        b.lane_ch[k] = c;
-        b.lane_ch[64 + k] = j;
-    }
-    const uint64_t core = __ballot(s >= 0);
-    const uint64_t chl = __ballot(c >= 0);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) top = max(top, __shfl_xor(top, o, 64));
-    if (s >= 0) {
-        const uint64_t bit = 1ull << k;
-        const int pos = __popcll(core & (bit - 1));
-        b.vis[s] = bit;
-        b.mb0[s] = bit;
-        sg_put<SM>(b, (int64_t)s * 64 + k, SM == SIG_F64 ? d2u(1.0) : 1ull);
-        b.Q[pos] = s;
-        b.Qm[pos] = bit;
-        if (b.lvl) b.lvl[(int64_t)s * 64 + k] = 0;
-        if (b.n_chain > 0 && b.ep_idx[s] >= 0) b.ep_lvl[(int64_t)b.ep_idx[s] * 64 + k] = 0;
-    }
     if (k == 0) {
-        b.ctr[0] = __popcll(core);
+        b.ctr[0] = base;
         b.ctr[1] = 0;
         b.ctr[2] = 0;
         b.ctr[3] = -1;
-        b.ctr[4] = top;  // last level with an injected chain-source lane
+        b.ctr[4] = top;  // last level with an injected chain source
         b.ctr[5] = ns;
         b.qoff[0] = 0;
-        if (b.n_chain > 0) *b.lane_mask = chl;
+        if (b.n_chain > 0)
+#pragma unroll
+            for (int h = 0; h < W; ++h) b.lane_mask[h] = chl[h];
     }
 }
 
 // ------------------------------------------------------------- expand ---
-__device__ __forceinline__ void push(const BatchDev &b, uint64_t *dst, int w, uint64_t F) {
-    const uint64_t m = F & ~b.vis[w];
-    if (m) atomicOr((unsigned long long *)&dst[w], (unsigned long long)m);  // result unused: no return
+template <int W>
+__device__ __forceinline__ void push(const BatchDev &b, uint64_t *dst, int w, const Msk<W> &F) {
+    mor<W>(dst, w, mandn<W>(F, mld<W>(b.vis, w)));
 }
 
-// Whether some chain-source lane still has an injection due after level l
-// into an endpoint it has not reached (vis is not written during an expand
-// launch, so every wave of the launch finds the same answer).
+// Whether some chain source still has an injection due after level l into
+// an endpoint it has not reached (vis is not written during an expand launch,
+// so every wave of the launch finds the same answer).
+template <int W>
 __device__ __forceinline__ bool seed_pending(const BatchDev &b, int l) {
     if (b.n_chain == 0) return false;
     const int lane = threadIdx.x & 63;
-    const int c = b.lane_ch[lane];
     bool p = false;
-    if (c >= 0) {
-        const int j = b.lane_ch[64 + lane], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
-        const uint64_t bit = 1ull << lane;
-        p = (j > l && !(b.vis[b.ch_L[c]] & bit)) || (K - j > l && !(b.vis[b.ch_R[c]] & bit));
+#pragma unroll
+    for (int h = 0; h < W; ++h) {
+        const int i = h * 64 + lane;
+        const int c = b.lane_ch[i];
+        if (c < 0) continue;
+        const int j = b.lane_ch[64 * W + i], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
+        p |= (j > l && !((b.vis[(int64_t)b.ch_L[c] * W + h] >> lane) & 1)) ||
+             (K - j > l && !((b.vis[(int64_t)b.ch_R[c] * W + h] >> lane) & 1));
     }
     return __ballot(p) != 0;
 }
 
-// Level l: Q_l = Q[qoff[l], ctr[0]).  Records qoff[l+1]; an empty Q_l ends the
-// forward with depth l-1.  Clears Q_{l-1} from mask array (l+2)%3.
+// Level l: Q_l = Q[qoff[l], ctr[0]).  Records qoff[l+1]; an empty Q_l with
+// no pending injection ends the forward with depth l-1.  Clears Q_{l-1} from
+// mask array (l+2)%3.
+template <int W>
 __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
     if (ldi(&b.ctr[3]) >= 0) return;
     const int q0 = b.qoff[l];
@@ -192,7 +330,7 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
     // an empty level ends the forward; so do full lists and a count that left
     // the sigma word (the host reruns the tile either way)
     bool stop = ldi(&b.ctr[2]) != 0 || ldi(&b.ctr[1]) != 0;
-    if (!stop && q1 == q0) stop = l >= b.ctr[4] || !seed_pending(b, l);
+    if (!stop && q1 == q0) stop = l >= b.ctr[4] || !seed_pending<W>(b, l);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         b.qoff[l + 1] = q1;
         if (stop) b.ctr[3] = l - 1;
@@ -201,25 +339,29 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
     uint64_t *dst = mbuf(b, l + 1);
     if (l > 0) {
         uint64_t *clr = mbuf(b, l + 2);
-        for (int i = b.qoff[l - 1] + blockIdx.x * BB + threadIdx.x; i < q0; i += gridDim.x * BB) clr[b.Q[i]] = 0;
+        for (int i = b.qoff[l - 1] + blockIdx.x * BB + threadIdx.x; i < q0; i += gridDim.x * BB)
+            mst<W>(clr, b.Q[i], mzero<W>());
     }
     const int lane = threadIdx.x & 63;
     if (b.n_chain > 0 && blockIdx.x == 0 && threadIdx.x < 64) {
-        // chain-source lanes whose endpoints are due at level l+1
-        const int c = b.lane_ch[lane];
-        if (c >= 0) {
-            const int j = b.lane_ch[64 + lane], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
-            if (j == l + 1) push(b, dst, b.ch_L[c], 1ull << lane);
-            if (K - j == l + 1) push(b, dst, b.ch_R[c], 1ull << lane);
+        // chain sources whose endpoints are due at level l+1
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+            const int i = h * 64 + lane;
+            const int c = b.lane_ch[i];
+            if (c < 0) continue;
+            const int j = b.lane_ch[64 * W + i], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
+            if (j == l + 1) push<W>(b, dst, b.ch_L[c], mbit<W>(h, lane));
+            if (K - j == l + 1) push<W>(b, dst, b.ch_R[c], mbit<W>(h, lane));
         }
     }
     for (int base = q0 + gwave() * 64; base < q1; base += nwaves() * 64) {
         const int e = base + lane;
         int j0 = 0, j1 = 0;
-        uint64_t F = 0;
+        Msk<W> F = mzero<W>();
         if (e < q1) {
             const int v = b.Q[e];
-            F = b.Qm[e];
+            F = mld<W>(b.Qm, e);
             j0 = b.rp[v];
             j1 = b.rp[v + 1];
         }
@@ -227,91 +369,99 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
         if (d <= SMALL)
             for (int j = j0; j < j1; ++j) {
                 const int w = b.col[j];
-                if (w >= 0) push(b, dst, w, F);  // (w < 0: a contracted chain)
+                if (w >= 0) push<W>(b, dst, w, F);  // (w < 0: a contracted chain)
             }
         uint64_t bal = __ballot(d > SMALL && !is_hub(b, d));
         while (bal) {
             const int src = __ffsll((long long)bal) - 1;
             bal &= bal - 1;
-            const uint64_t FF = shfl64(F, src);
+            const Msk<W> FF = mshfl<W>(F, src);
             const int a0 = __shfl(j0, src, 64), a1 = __shfl(j1, src, 64);
             for (int j = a0 + lane; j < a1; j += 64) {
                 const int w = b.col[j];
-                if (w >= 0) push(b, dst, w, FF);
+                if (w >= 0) push<W>(b, dst, w, FF);
             }
         }
     }
-    // hub slices: the hub's level-l lanes are its mask in mb[l%3]
+    // hub slices: the hub's level-l sources are its mask in mb[l%3]
     const uint64_t *cur = mbuf(b, l);
     for (int s = gwave(); s < b.n_slice; s += nwaves()) {
         const int v = b.hub_v[b.sl_hub[s]];
-        const uint64_t F = cur[v];
-        if (!F) continue;
+        const Msk<W> F = mld<W>(cur, v);
+        if (!many<W>(F)) continue;
         for (int j = b.sl_j0[s] + lane; j < b.sl_j1[s]; j += 64) {
             const int w = b.col[j];
-            if (w >= 0) push(b, dst, w, F);
+            if (w >= 0) push<W>(b, dst, w, F);
         }
     }
 }
 
 // --------------------------------------------------------------- pull ---
-// sigma_w for the lanes N of entry w: the sum over neighbours at level l
-// (fr) in CSR order.  A contracted chain's arc (~c) counts 1 for the lane
-// whose source lies on chain c and is injected at w on this level (sc == c);
-// cm holds the tile's chain-source lanes.
-template <int SM>
-__device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, uint64_t N, int a0, int a1,
-                                          uint64_t &acc, double &facc, bool &ovf, int sc, uint64_t cm) {
+// sigma_w for the sources N of entry w: the sum over neighbours at level l
+// (fr) in CSR order.  A contracted chain's arc (~c) counts 1 for the source
+// of word h whose chain is c and that is injected at w on this level
+// (sc[h] == c); cm holds the tile's chain sources.
+template <int SM, int W>
+__device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, const Msk<W> &N, int a0, int a1,
+                                          uint64_t (&acc)[W], double (&facc)[W], bool &ovf, const int (&sc)[W],
+                                          const Msk<W> &cm) {
     const int lane = threadIdx.x & 63;
-    const uint64_t me = 1ull << lane;
     for (int c = a0; c < a1; c += 64) {
         const int j = c + lane;
         int v = 0;
-        uint64_t f = 0;
+        Msk<W> f = mzero<W>();
         if (j < a1) {
             v = b.col[j];
-            f = v >= 0 ? fr[v] & N : N & cm;
+            f = v >= 0 ? mand<W>(mld<W>(fr, v), N) : mand<W>(N, cm);
         }
-        uint64_t bal = __ballot(f != 0);
+        uint64_t bal = __ballot(many<W>(f));
         while (bal) {
-            uint64_t x[UNR];
-            bool use[UNR];
+            uint64_t x[UNR][W];
+            int use[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
-                use[u] = false;
-                x[u] = 0;
+                use[u] = 0;
+#pragma unroll
+                for (int h = 0; h < W; ++h) x[u][h] = 0;
                 if (bal) {
                     const int src = __ffsll((long long)bal) - 1;
                     bal &= bal - 1;
                     const int vv = __shfl(v, src, 64);
-                    const uint64_t ff = shfl64(f, src);
-                    if (ff & me) {
+                    const int pb = mbits<W>(mshfl<W>(f, src), lane);
+                    if (pb) {
                         if (vv >= 0) {
-                            use[u] = true;
-                            x[u] = sg<SM>(b, (int64_t)vv * 64 + lane);
-                        } else if (~vv == sc) {
-                            use[u] = true;
-                            x[u] = SM == SIG_F64 ? d2u(1.0) : 1ull;
+                            use[u] = pb;
+                            sg_row<SM, W>(b, (int64_t)vv * 64 + lane, x[u]);
+                        } else {
+#pragma unroll
+                            for (int h = 0; h < W; ++h)
+                                if (((pb >> h) & 1) && ~vv == sc[h]) {
+                                    use[u] |= 1 << h;
+                                    x[u][h] = SM == SIG_F64 ? d2u(1.0) : 1ull;
+                                }
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                if (!use[u]) continue;
-                if (SM == SIG_F64) {
-                    facc += u2d(x[u]);
-                } else {
-                    const uint64_t s = acc + x[u];
-                    ovf |= (s < acc);
-                    acc = s;
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int h = 0; h < W; ++h) {
+                    if (!((use[u] >> h) & 1)) continue;
+                    if (SM == SIG_F64) {
+                        facc[h] += u2d(x[u][h]);
+                    } else {
+                        const uint64_t s = acc[h] + x[u][h];
+                        ovf |= (s < acc[h]);
+                        acc[h] = s;
+                    }
                 }
-            }
         }
     }
 }
 
-__device__ __forceinline__ void stage_flush(const BatchDev &b, int *sw, uint64_t *sm, int &cnt, int lane) {
+template <int W>
+__device__ __forceinline__ void stage_flush(const BatchDev &b, int *sw, Msk<W> *sm, int &cnt, int lane) {
     if (cnt == 0) return;
     int base = 0;
     if (lane == 0) base = atomicAdd(&b.ctr[0], cnt);
@@ -321,7 +471,7 @@ __device__ __forceinline__ void stage_flush(const BatchDev &b, int *sw, uint64_t
         const int pos = base + i;
         if (pos < b.qcap) {
             b.Q[pos] = sw[i];
-            b.Qm[pos] = sm[i];
+            mst<W>(b.Qm, pos, sm[i]);
         } else {
             b.ctr[2] = 1;
         }
@@ -331,39 +481,43 @@ __device__ __forceinline__ void stage_flush(const BatchDev &b, int *sw, uint64_t
 }
 
 // Builds Q_{l+1} from the non-zero entries of mb[(l+1)%3] and their sigma.
-template <int SM>
+template <int SM, int W>
 __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
     constexpr bool F64 = SM == SIG_F64;
     __shared__ int s_w[NW][STAGE];
-    __shared__ uint64_t s_m[NW][STAGE];
+    __shared__ Msk<W> s_m[NW][STAGE];
     if (ldi(&b.ctr[3]) >= 0) return;
     const uint64_t *fr = mbuf(b, l);
     const uint64_t *nx = mbuf(b, l + 1);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t me = 1ull << lane;
     const bool split = !F64 && b.n_hub > 0;
-    // chain-source lane: its endpoints injected at level l+1 (else -1)
-    int myc = -1, mL = -1, mR = -1;
-    uint64_t cm = 0;
-    if (b.n_chain > 0) {
-        cm = *b.lane_mask;
-        myc = b.lane_ch[lane];
-        if (myc >= 0) {
-            const int j = b.lane_ch[64 + lane], K = b.ch_off[myc + 1] - b.ch_off[myc] + 1;
-            if (j == l + 1) mL = b.ch_L[myc];
-            if (K - j == l + 1) mR = b.ch_R[myc];
-        }
+    // chain sources: their chain and their endpoints injected at level l+1
+    // (else -1)
+    int myc[W], mL[W], mR[W];
+    Msk<W> cm = mzero<W>();
+#pragma unroll
+    for (int h = 0; h < W; ++h) {
+        myc[h] = mL[h] = mR[h] = -1;
+        if (b.n_chain == 0) continue;
+        cm.m[h] = b.lane_mask[h];
+        const int i = h * 64 + lane;
+        myc[h] = b.lane_ch[i];
+        if (myc[h] < 0) continue;
+        const int j = b.lane_ch[64 * W + i], K = b.ch_off[myc[h] + 1] - b.ch_off[myc[h]] + 1;
+        if (j == l + 1) mL[h] = b.ch_L[myc[h]];
+        if (K - j == l + 1) mR[h] = b.ch_R[myc[h]];
     }
     int cnt = 0;  // wave-uniform
     for (int base = gwave() * 64; base < b.n; base += nwaves() * 64) {
         const int w0 = base + lane;
-        const uint64_t N0 = w0 < b.n ? nx[w0] : 0;
-        const uint64_t bal = __ballot(N0 != 0);
+        const Msk<W> N0 = w0 < b.n ? mld<W>(nx, w0) : mzero<W>();
+        const uint64_t bal = __ballot(many<W>(N0));
         if (!bal) continue;
         const int k0 = cnt;
         // CSR ranges of the chunk's entries, loaded lane-parallel up front
         int r0 = 0, r1 = 0, e0 = -1;
-        if (N0) {
+        if (many<W>(N0)) {
             const int slot = k0 + __popcll(bal & (me - 1));
             s_w[wv][slot] = w0;
             s_m[wv][slot] = N0;
@@ -378,52 +532,80 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
             const int src = __ffsll((long long)rest) - 1;
             rest &= rest - 1;
             const int w = __shfl(w0, src, 64);
-            const uint64_t N = shfl64(N0, src);
+            const Msk<W> N = mshfl<W>(N0, src);
             const int a0 = __shfl(r0, src, 64), a1 = __shfl(r1, src, 64);
             const int ew = __shfl(e0, src, 64);
+            const int pb = mbits<W>(N, lane);
+            const int64_t cell = ((int64_t)w * 64 + lane) * W;
             if (!(split && is_hub(b, a1 - a0))) {
-                uint64_t acc = 0;
-                double facc = 0.0;
+                uint64_t acc[W];
+                double facc[W];
+                int sc[W];
+#pragma unroll
+                for (int h = 0; h < W; ++h) {
+                    acc[h] = 0;
+                    facc[h] = 0.0;
+                    sc[h] = (w == mL[h] || w == mR[h]) ? myc[h] : -1;
+                }
                 bool ovf = false;
-                sigma_sum<SM>(b, fr, N, a0, a1, acc, facc, ovf, (w == mL || w == mR) ? myc : -1, cm);
-                if (N & me) {
-                    sg_put<SM>(b, (int64_t)w * 64 + lane, F64 ? d2u(facc) : acc);
-                    if (!F64 && (ovf || acc > sg_max<SM>() || (b.limit && acc >= b.limit))) b.ctr[1] = 1;
+                sigma_sum<SM, W>(b, fr, N, a0, a1, acc, facc, ovf, sc, cm);
+#pragma unroll
+                for (int h = 0; h < W; ++h) {
+                    if (!((pb >> h) & 1)) continue;
+                    sg_put<SM>(b, cell + h, F64 ? d2u(facc[h]) : acc[h]);
+                    if (!F64 && (ovf || acc[h] > sg_max<SM>() || (b.limit && acc[h] >= b.limit))) b.ctr[1] = 1;
                 }
             }
-            if ((N & me) && b.lvl) b.lvl[(int64_t)w * 64 + lane] = l + 1;
-            if ((N & me) && ew >= 0) b.ep_lvl[(int64_t)ew * 64 + lane] = l + 1;
-            if (lane == 0) b.vis[w] |= N;
+#pragma unroll
+            for (int h = 0; h < W; ++h) {
+                if (!((pb >> h) & 1)) continue;
+                if (b.lvl) b.lvl[cell + h] = l + 1;
+                if (ew >= 0) b.ep_lvl[((int64_t)ew * 64 + lane) * W + h] = l + 1;
+            }
+            if (lane == 0)
+#pragma unroll
+                for (int h = 0; h < W; ++h) b.vis[(int64_t)w * W + h] |= N.m[h];
         }
-        if (cnt >= STAGE_FLUSH) stage_flush(b, s_w[wv], s_m[wv], cnt, lane);
+        if (cnt >= STAGE_FLUSH) stage_flush<W>(b, s_w[wv], s_m[wv], cnt, lane);
     }
-    stage_flush(b, s_w[wv], s_m[wv], cnt, lane);
+    stage_flush<W>(b, s_w[wv], s_m[wv], cnt, lane);
     // hub slices: the level mask of every hub is recorded for the backward;
     // partial sigma sums are added atomically (exact integers)
     for (int s = gwave(); s < b.n_slice; s += nwaves()) {
-        const int h = b.sl_hub[s];
-        const int v = b.hub_v[h];
-        const uint64_t N = nx[v];
+        const int hb = b.sl_hub[s];
+        const int v = b.hub_v[hb];
+        const Msk<W> N = mld<W>(nx, v);
         if (b.sl_j0[s] == b.rp[v] && lane == 0 && l + 1 < b.hub_stride)
-            b.hub_mask[(int64_t)h * b.hub_stride + l + 1] = N;
-        if (!split || !N) continue;
-        uint64_t acc = 0;
-        double facc = 0.0;
+            mst<W>(b.hub_mask, (int64_t)hb * b.hub_stride + l + 1, N);
+        if (!split || !many<W>(N)) continue;
+        uint64_t acc[W];
+        double facc[W];
+        int sc[W];
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+            acc[h] = 0;
+            facc[h] = 0.0;
+            sc[h] = (v == mL[h] || v == mR[h]) ? myc[h] : -1;
+        }
         bool ovf = false;
         constexpr int SI = F64 ? SIG_U64 : SM;  // (not reached in FP64 mode)
-        sigma_sum<SI>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf, (v == mL || v == mR) ? myc : -1, cm);
-        if ((N & me) && acc) {
-            const int64_t k = (int64_t)v * 64 + lane;
+        sigma_sum<SI, W>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf, sc, cm);
+        const int pb = mbits<W>(N, lane);
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+            if (!((pb >> h) & 1) || !acc[h]) continue;
+            const int64_t k = ((int64_t)v * 64 + lane) * W + h;
             uint64_t old, nw;
+            bool o2 = ovf;
             if (SI == SIG_U32) {
-                old = atomicAdd((unsigned int *)b.sig + k, (unsigned int)acc);
-                nw = old + acc;
+                old = atomicAdd((unsigned int *)b.sig + k, (unsigned int)acc[h]);
+                nw = old + acc[h];
             } else {
-                old = atomicAdd((unsigned long long *)b.sig + k, (unsigned long long)acc);
-                nw = old + acc;
-                ovf |= nw < old;
+                old = atomicAdd((unsigned long long *)b.sig + k, (unsigned long long)acc[h]);
+                nw = old + acc[h];
+                o2 |= nw < old;
             }
-            if (ovf || acc > sg_max<SI>() || nw > sg_max<SI>() || (b.limit && nw >= b.limit)) b.ctr[1] = 1;
+            if (o2 || acc[h] > sg_max<SI>() || nw > sg_max<SI>() || (b.limit && nw >= b.limit)) b.ctr[1] = 1;
         }
     }
 }
@@ -485,13 +667,15 @@ __device__ __forceinline__ double seg_term(int a, int bl, double sA, double sB, 
     return term;
 }
 
-// The chain terms of endpoint e (vertex at level l, count sv) for this lane:
-// every attached chain is a segment from e to its other endpoint, except the
-// chain holding the lane's own source x_j, which is the segment from e to the
-// source (level 0, count 1, j arcs from L or K - j from R).
-template <int SM>
-__device__ double chain_rho(const BatchDev &b, int e, int l, double sv, int lane) {
-    const int myc = b.lane_ch[lane], myj = b.lane_ch[64 + lane];
+// The chain terms of endpoint e (vertex at level l, count sv) for source
+// slot i = h * 64 + lane: every attached chain is a segment from e to its
+// other endpoint, except the chain holding the source x_j itself, which is the
+// segment from e to the source (level 0, count 1, j arcs from L or K - j from
+// R).
+template <int SM, int W>
+__device__ double chain_rho(const BatchDev &b, int e, int l, double sv, int lane, int h) {
+    const int i = h * 64 + lane;
+    const int myc = b.lane_ch[i], myj = b.lane_ch[64 * W + i];
     double rho = 0.0;
     for (int q = b.ep_off[e]; q < b.ep_off[e + 1]; ++q) {
         const int code = b.ep_list[q];
@@ -502,70 +686,75 @@ __device__ double chain_rho(const BatchDev &b, int e, int l, double sv, int lane
             rho += seg_term(l, 0, sv, 1.0, side ? K - myj : myj, P);
         } else {
             const int o = side ? b.ch_L[c] : b.ch_R[c];
-            const int lv = b.ep_lvl[(int64_t)b.ep_idx[o] * 64 + lane];
-            const double so = sg_val<SM>(sg<SM>(b, (int64_t)o * 64 + lane));
+            const int lv = b.ep_lvl[((int64_t)b.ep_idx[o] * 64 + lane) * W + h];
+            const double so = sg_val<SM>(sg<SM>(b, ((int64_t)o * 64 + lane) * W + h));
             rho += seg_term(l, lv, sv, so, K, P);
         }
     }
     return rho;
 }
 
-// rho_v = sum over successors w (lanes in succ[w] & M) of t_w.
-template <typename TT>
-__device__ __forceinline__ double rho_sum(const BatchDev &b, const uint64_t *succ, uint64_t M, int a0, int a1) {
+// rho_v = sum over successors w (sources in succ[w] & M) of t_w.
+template <typename TT, int W>
+__device__ __forceinline__ void rho_sum(const BatchDev &b, const uint64_t *succ, const Msk<W> &M, int a0, int a1,
+                                        double (&out)[W]) {
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
-    const TT *tt = (const TT *)b.tt;
     const int lane = threadIdx.x & 63;
-    const uint64_t me = 1ull << lane;
-    Acc acc;
+    Acc acc[W];
     for (int c = a0; c < a1; c += 64) {
         const int j = c + lane;
         int w = 0;
-        uint64_t x = 0;
+        Msk<W> x = mzero<W>();
         if (j < a1) {
             w = b.col[j];
-            x = w >= 0 ? succ[w] & M : 0;
+            if (w >= 0) x = mand<W>(mld<W>(succ, w), M);
         }
-        uint64_t bal = __ballot(x != 0);
+        uint64_t bal = __ballot(many<W>(x));
         while (bal) {
-            TT t[UNR];
-            bool use[UNR];
+            TT t[UNR][W];
+            int use[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
-                use[u] = false;
-                t[u] = 0;
+                use[u] = 0;
+#pragma unroll
+                for (int h = 0; h < W; ++h) t[u][h] = 0;
                 if (bal) {
                     const int src = __ffsll((long long)bal) - 1;
                     bal &= bal - 1;
                     const int ww = __shfl(w, src, 64);
-                    const uint64_t xx = shfl64(x, src);
-                    if (xx & me) {
-                        use[u] = true;
-                        t[u] = tt[(int64_t)ww * 64 + lane];
-                    }
+                    use[u] = mbits<W>(mshfl<W>(x, src), lane);
+                    if (use[u]) tt_row<TT, W>(b, (int64_t)ww * 64 + lane, t[u]);
                 }
             }
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
-                if (use[u]) acc.add(t[u]);
+#pragma unroll
+                for (int h = 0; h < W; ++h)
+                    if ((use[u] >> h) & 1) acc[h].add(t[u][h]);
         }
     }
-    return acc.value();
+#pragma unroll
+    for (int h = 0; h < W; ++h) out[h] = acc[h].value();
 }
 
-// rho complete for (v, lanes M): delta, BC contribution and the term t_v.
-template <int SM, typename TT>
-__device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, int l, uint64_t M, double acc) {
+// rho complete for (v, sources M): delta, BC contribution and the term t_v.
+template <int SM, typename TT, int W>
+__device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, int l, const Msk<W> &M, double (&acc)[W]) {
     const int lane = threadIdx.x & 63;
     const int e = b.n_chain > 0 ? b.ep_idx[v] : -1;
+    const int pb = mbits<W>(M, lane);
     double contrib = 0.0;
-    if (M & (1ull << lane)) {
-        const double sv = sg_val<SM>(sg<SM>(b, (int64_t)v * 64 + lane));
-        if (e >= 0) acc += chain_rho<SM>(b, e, l, sv, lane);
-        const double delta = sv * acc;
-        contrib = b.wsl[lane] * delta;
-        ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + acc);
-        if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
+#pragma unroll
+    for (int h = 0; h < W; ++h) {
+        if (!((pb >> h) & 1)) continue;
+        const int64_t cell = ((int64_t)v * 64 + lane) * W + h;
+        const double sv = sg_val<SM>(sg<SM>(b, cell));
+        double a = acc[h];
+        if (e >= 0) a += chain_rho<SM, W>(b, e, l, sv, lane, h);
+        const double delta = sv * a;
+        contrib += b.wsl[h * 64 + lane] * delta;
+        ((TT *)b.tt)[cell] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
+        if (b.dbg_delta) b.dbg_delta[cell] = delta;
     }
     contrib = wave_sum(contrib);
     // b.bc is the slot's own accumulator and v is finished once per level
@@ -575,13 +764,15 @@ __device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, int l, u
 
 // Level l <= depth: entries Q_l; Q_{l+2} is cleared from mb[(l+2)%3] and Q_l
 // scattered into mb[l%3] for level l-1.
-template <int SM, typename TT>
+template <int SM, typename TT, int W>
 __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     constexpr bool F64 = SM == SIG_F64;
     __shared__ int t_w[NW][WIN];
-    __shared__ uint64_t t_x[NW][WIN];
+    __shared__ Msk<W> t_x[NW][WIN];
     __shared__ int t_k[NW][WIN];
-    __shared__ double r_rho[NW][FEB][64];
+    // entries per wave block: FEB / W, so a block holds FEB * 64 (entry, source) pairs
+    constexpr int EB = FEB / W;
+    __shared__ double r_rho[NW][EB][64 * W];  // [entry][h * 64 + lane]
     // the host launches the backward speculatively behind the forward: a
     // level beyond the depth, an unfinished forward (depth -1), a sigma
     // overflow (the tile is redone in FP64) or full lists all mean no work
@@ -592,29 +783,28 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     uint64_t *own = mbuf(b, l);
     const uint64_t *succ = mbuf(b, l + 1);
     uint64_t *clr = mbuf(b, l + 2);
-    for (int i = c0 + blockIdx.x * BB + threadIdx.x; i < c1; i += gridDim.x * BB) clr[b.Q[i]] = 0;
+    for (int i = c0 + blockIdx.x * BB + threadIdx.x; i < c1; i += gridDim.x * BB) mst<W>(clr, b.Q[i], mzero<W>());
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t me = 1ull << lane;
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
-    const TT *tt = (const TT *)b.tt;
-    // Blocks of FEB list entries per wave.  The arcs of the block are walked
+    // Blocks of EB list entries per wave.  The arcs of the block are walked
     // as one flat range (lane per arc) in windows of WIN: successor masks
-    // are gathered for 64 arcs at once and the (entry, successor, lanes)
+    // are gathered for 64 arcs at once and the (entry, successor, sources)
     // triples that matter are compacted in LDS; the t rows of those triples
     // are then gathered UNR at a time, accumulated per entry in arc order.
-    for (int base = q0 + gwave() * FEB; base < q1; base += nwaves() * FEB) {
+    for (int base = q0 + gwave() * EB; base < q1; base += nwaves() * EB) {
         const int e = base + lane;
         int v0 = 0, r0 = 0, r1 = 0, ep0 = -1;
-        uint64_t M0 = 0;
-        if (lane < FEB && e < q1) {
+        Msk<W> M0 = mzero<W>();
+        if (lane < EB && e < q1) {
             v0 = b.Q[e];
-            M0 = b.Qm[e];
-            own[v0] = M0;
+            M0 = mld<W>(b.Qm, e);
+            mst<W>(own, v0, M0);
             r0 = b.rp[v0];
             r1 = b.rp[v0 + 1];
             if (b.n_chain > 0) ep0 = b.ep_idx[v0];
         }
-        const int cnt = min(FEB, q1 - base);
+        const int cnt = min(EB, q1 - base);
         const bool mine = lane < cnt && !(is_hub(b, r1 - r0) && l < b.hub_stride);  // hubs: slices
         const int d = mine ? r1 - r0 : 0;
         int inc = d;
@@ -625,20 +815,21 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         }
         const int pre = inc - d;
         const int T = __shfl(inc, 63, 64);
-        int P[FEB], A[FEB], V[FEB], EP[FEB];
-        uint64_t MM[FEB];
-        bool OWN[FEB];
+        int P[EB], A[EB], V[EB], EP[EB];
+        Msk<W> MM[EB];
+        bool OWN[EB];
 #pragma unroll
-        for (int j = 0; j < FEB; ++j) {
+        for (int j = 0; j < EB; ++j) {
             P[j] = j < cnt ? __shfl(pre, j, 64) : INT32_MAX;
             A[j] = __shfl(r0, j, 64);
             V[j] = __shfl(v0, j, 64);
             EP[j] = __shfl(ep0, j, 64);
-            MM[j] = shfl64(M0, j);
+            MM[j] = mshfl<W>(M0, j);
             OWN[j] = __shfl((int)mine, j, 64) != 0;
-            r_rho[wv][j][lane] = 0.0;
+#pragma unroll
+            for (int h = 0; h < W; ++h) r_rho[wv][j][h * 64 + lane] = 0.0;
         }
-        Acc acc;
+        Acc acc[W];
         int kcur = -1;
         for (int win = 0; win < T; win += WIN) {
             const int wend = min(T, win + WIN);
@@ -647,20 +838,24 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                 const int g = c + lane;
                 int k = 0;
 #pragma unroll
-                for (int j = 1; j < FEB; ++j) k += (P[j] <= g);
+                for (int j = 1; j < EB; ++j) k += (P[j] <= g);
                 int a0k = A[0], pk = P[0];
-                uint64_t mk = MM[0];
+                Msk<W> mk = MM[0];
 #pragma unroll
-                for (int j = 1; j < FEB; ++j)
-                    if (k == j) { a0k = A[j]; pk = P[j]; mk = MM[j]; }
+                for (int j = 1; j < EB; ++j)
+                    if (k == j) {
+                        a0k = A[j];
+                        pk = P[j];
+                        mk = MM[j];
+                    }
                 int w = 0;
-                uint64_t x = 0;
+                Msk<W> x = mzero<W>();
                 if (g < wend) {
                     w = b.col[a0k + g - pk];
-                    x = w >= 0 ? succ[w] & mk : 0;  // (w < 0: a contracted chain)
+                    if (w >= 0) x = mand<W>(mld<W>(succ, w), mk);  // (w < 0: a contracted chain)
                 }
-                const uint64_t bal = __ballot(x != 0);
-                if (x) {
+                const uint64_t bal = __ballot(many<W>(x));
+                if (many<W>(x)) {
                     const int pos = nrel + __popcll(bal & (me - 1));
                     t_w[wv][pos] = w;
                     t_x[wv][pos] = x;
@@ -670,54 +865,67 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
             }
             __builtin_amdgcn_wave_barrier();
             for (int r = 0; r < nrel; r += UNR) {
-                TT t[UNR];
-                int kk[UNR];
-                bool use[UNR];
+                TT t[UNR][W];
+                int kk[UNR], use[UNR];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     const int rr = r + u;
-                    use[u] = false;
+                    use[u] = 0;
                     kk[u] = -1;
-                    t[u] = 0;
+#pragma unroll
+                    for (int h = 0; h < W; ++h) t[u][h] = 0;
                     if (rr < nrel) {
                         kk[u] = t_k[wv][rr];
-                        if (t_x[wv][rr] & me) {
-                            use[u] = true;
-                            t[u] = tt[(int64_t)t_w[wv][rr] * 64 + lane];
-                        }
+                        use[u] = mbits<W>(t_x[wv][rr], lane);
+                        if (use[u]) tt_row<TT, W>(b, (int64_t)t_w[wv][rr] * 64 + lane, t[u]);
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     if (kk[u] < 0) continue;
                     if (kk[u] != kcur) {
-                        if (kcur >= 0) r_rho[wv][kcur][lane] = acc.value();
-                        acc = Acc();
+                        if (kcur >= 0)
+#pragma unroll
+                            for (int h = 0; h < W; ++h) r_rho[wv][kcur][h * 64 + lane] = acc[h].value();
+#pragma unroll
+                        for (int h = 0; h < W; ++h) acc[h] = Acc();
                         kcur = kk[u];
                     }
-                    if (use[u]) acc.add(t[u]);
+#pragma unroll
+                    for (int h = 0; h < W; ++h)
+                        if ((use[u] >> h) & 1) acc[h].add(t[u][h]);
                 }
             }
             __builtin_amdgcn_wave_barrier();
         }
-        if (kcur >= 0) r_rho[wv][kcur][lane] = acc.value();
+        if (kcur >= 0)
+#pragma unroll
+            for (int h = 0; h < W; ++h) r_rho[wv][kcur][h * 64 + lane] = acc[h].value();
         // finish every entry of the block (hubs excepted): sigma rows first
-        uint64_t SX[FEB];
+        uint64_t SX[EB][W];
 #pragma unroll
-        for (int j = 0; j < FEB; ++j) SX[j] = (OWN[j] && (MM[j] & me)) ? sg<SM>(b, (int64_t)V[j] * 64 + lane) : 0;
+        for (int j = 0; j < EB; ++j) {
 #pragma unroll
-        for (int j = 0; j < FEB; ++j) {
+            for (int h = 0; h < W; ++h) SX[j][h] = 0;
+            if (OWN[j] && mbits<W>(MM[j], lane)) sg_row<SM, W>(b, (int64_t)V[j] * 64 + lane, SX[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < EB; ++j) {
             if (!OWN[j]) continue;
             const int v = V[j];
+            const int pb = mbits<W>(MM[j], lane);
             double contrib = 0.0;
-            if (MM[j] & me) {
-                const double sv = sg_val<SM>(SX[j]);
-                double a = r_rho[wv][j][lane];  // rho_v
-                if (EP[j] >= 0) a += chain_rho<SM>(b, EP[j], l, sv, lane);
+#pragma unroll
+            for (int h = 0; h < W; ++h) {
+                if (!((pb >> h) & 1)) continue;
+                const int64_t cell = ((int64_t)v * 64 + lane) * W + h;
+                const double sv = sg_val<SM>(SX[j][h]);
+                double a = r_rho[wv][j][h * 64 + lane];  // rho_v
+                if (EP[j] >= 0) a += chain_rho<SM, W>(b, EP[j], l, sv, lane, h);
                 const double delta = sv * a;
-                contrib = b.wsl[lane] * delta;
-                ((TT *)b.tt)[(int64_t)v * 64 + lane] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
-                if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = delta;
+                contrib += b.wsl[h * 64 + lane] * delta;
+                ((TT *)b.tt)[cell] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
+                if (b.dbg_delta) b.dbg_delta[cell] = delta;
             }
             contrib = wave_sum(contrib);
             if (lane == 0 && contrib != 0.0) b.bc[v] += contrib;
@@ -727,21 +935,30 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     // accumulators; the last slice of a hub to finish (counter) completes it
     if (l >= b.hub_stride) return;
     for (int s = gwave(); s < b.n_slice; s += nwaves()) {
-        const int h = b.sl_hub[s];
-        const uint64_t M = b.hub_mask[(int64_t)h * b.hub_stride + l];
-        if (!M) continue;
-        const int v = b.hub_v[h];
-        const double part = rho_sum<TT>(b, succ, M, b.sl_j0[s], b.sl_j1[s]);
-        if ((M & (1ull << lane)) && part != 0.0) unsafeAtomicAdd(&b.hub_acc[h * 64 + lane], part);
+        const int hb = b.sl_hub[s];
+        const Msk<W> M = mld<W>(b.hub_mask, (int64_t)hb * b.hub_stride + l);
+        if (!many<W>(M)) continue;
+        const int v = b.hub_v[hb];
+        double part[W];
+        rho_sum<TT, W>(b, succ, M, b.sl_j0[s], b.sl_j1[s], part);
+        const int pb = mbits<W>(M, lane);
+#pragma unroll
+        for (int h = 0; h < W; ++h)
+            if (((pb >> h) & 1) && part[h] != 0.0)
+                unsafeAtomicAdd(&b.hub_acc[((int64_t)hb * 64 + lane) * W + h], part[h]);
         __threadfence();
         int done = 0;
-        if (lane == 0) done = atomicAdd(&b.hub_cnt[h], 1);
+        if (lane == 0) done = atomicAdd(&b.hub_cnt[hb], 1);
         done = __shfl(done, 0, 64);
-        if (done != b.hub_nsl[h] - 1) continue;
+        if (done != b.hub_nsl[hb] - 1) continue;
         __threadfence();
-        const double rho = __hip_atomic_load(&b.hub_acc[h * 64 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        finish_vertex<SM, TT>(b, v, l, M, rho);
-        if (lane == 0) b.hub_cnt[h] = 0;
+        double rho[W];
+#pragma unroll
+        for (int h = 0; h < W; ++h)
+            rho[h] = __hip_atomic_load(&b.hub_acc[((int64_t)hb * 64 + lane) * W + h], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        finish_vertex<SM, TT, W>(b, v, l, M, rho);
+        if (lane == 0) b.hub_cnt[hb] = 0;
     }
 }
 
@@ -785,156 +1002,192 @@ __device__ __forceinline__ void seg_interior(int a, int bl, double sA, double sB
 // Between the forward and the backward: a tie interior's count sA + sB must
 // fit the tile's word (and stay below the test threshold), like every count
 // the pull makes; otherwise the tile is redone with the next word.
-template <int SM>
+template <int SM, int W>
 __global__ __launch_bounds__(BB) void k_b_chain_check(BatchDev b) {
     if (ldi(&b.ctr[3]) < 0 || b.ctr[1] || b.ctr[2]) return;
     const int lane = threadIdx.x & 63;
-    if (lane >= b.ctr[5]) return;
-    const int myc = b.lane_ch[lane], myj = b.lane_ch[64 + lane];
+    const int ns = b.ctr[5];
     for (int c = gwave(); c < b.n_chain; c += nwaves()) {
         const int L = b.ch_L[c], R = b.ch_R[c], K = b.ch_off[c + 1] - b.ch_off[c] + 1;
-        const int aL = b.ep_lvl[(int64_t)b.ep_idx[L] * 64 + lane];
-        const int aR = b.ep_lvl[(int64_t)b.ep_idx[R] * 64 + lane];
-        const uint64_t wL = sg<SM>(b, (int64_t)L * 64 + lane), wR = sg<SM>(b, (int64_t)R * 64 + lane);
-        // the segments of this lane: L .. R, or source .. L and source .. R;
-        // a tie needs an even S = b - a + K with 2 <= S <= 2K - 2
-        auto tie = [](int S, int k) { return (S & 1) == 0 && S >= 2 && S <= 2 * k - 2; };
-        auto check = [&](uint64_t x, uint64_t y) {
-            const uint64_t z = x + y;
-            if (z < x || z > sg_max<SM>() || (b.limit && z >= b.limit)) b.ctr[1] = 1;
-        };
-        if (c != myc) {
-            if (tie(aR - aL + K, K)) check(wL, wR);
-        } else {
-            if (tie(aL + myj, myj)) check(wL, 1);
-            if (tie(aR + K - myj, K - myj)) check(wR, 1);
+        const int64_t eL = (int64_t)b.ep_idx[L] * 64, eR = (int64_t)b.ep_idx[R] * 64;
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+            const int i = h * 64 + lane;
+            if (i >= ns) continue;
+            const int myc = b.lane_ch[i], myj = b.lane_ch[64 * W + i];
+            const int aL = b.ep_lvl[(eL + lane) * W + h], aR = b.ep_lvl[(eR + lane) * W + h];
+            const uint64_t wL = sg<SM>(b, ((int64_t)L * 64 + lane) * W + h);
+            const uint64_t wR = sg<SM>(b, ((int64_t)R * 64 + lane) * W + h);
+            // the segments of this source: L .. R, or source .. L and source ..
+            // R; a tie needs an even S = b - a + K with 2 <= S <= 2K - 2
+            auto tie = [](int S, int k) { return (S & 1) == 0 && S >= 2 && S <= 2 * k - 2; };
+            auto check = [&](uint64_t x, uint64_t y) {
+                const uint64_t z = x + y;
+                if (z < x || z > sg_max<SM>() || (b.limit && z >= b.limit)) b.ctr[1] = 1;
+            };
+            if (c != myc) {
+                if (tie(aR - aL + K, K)) check(wL, wR);
+            } else {
+                if (tie(aL + myj, myj)) check(wL, 1);
+                if (tie(aR + K - myj, K - myj)) check(wR, 1);
+            }
         }
     }
 }
 
 // After the backward: every contracted interior of every chain, one wave per
 // chain (the launch has a wave for every chain, so the chains' dependent
-// loads overlap) and one lane per source; BC into the slot's accumulator
-// (each interior has one writer), and in the probe the interiors' rows.  For a lane
-// whose source is off the chain, the segment L .. R's constants (split
-// point, tie, the prefix sums at the split) are set once per chain, so an
-// interior costs the chain's running prefix sum (the same word for every
-// lane), a few selects and the wave sum; the rare lane whose source lies on
-// the chain takes the general form for its two halves.
-template <int SM, typename TT>
+// loads overlap) and one lane per source slot; BC into the slot's
+// accumulator (each interior has one writer), and in the probe the
+// interiors' rows.  For a source off the chain, the segment L .. R's
+// constants (split point, tie, the prefix sums at the split) are set once
+// per chain, so an interior costs the chain's running prefix sum (the same
+// word for every lane), a few selects and the wave sum; the rare source that
+// lies on the chain takes the general form for its two halves.
+template <int SM, typename TT, int W>
 __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
     constexpr bool F64 = SM == SIG_F64;
     if (ldi(&b.ctr[3]) < 0 || (!F64 && b.ctr[1]) || b.ctr[2]) return;
     const int lane = threadIdx.x & 63;
-    const bool act = lane < b.ctr[5];
-    const double w = b.wsl[lane];
-    const int myc = b.lane_ch[lane], myj = b.lane_ch[64 + lane];
+    const int ns = b.ctr[5];
     const uint64_t one = F64 ? d2u(1.0) : 1ull;
     const TT *tt = (const TT *)b.tt;
+    bool act[W];
+    double wgt[W];
+    int myc[W], myj[W];
+#pragma unroll
+    for (int h = 0; h < W; ++h) {
+        const int i = h * 64 + lane;
+        act[h] = i < ns;
+        wgt[h] = b.wsl[i];
+        myc[h] = b.lane_ch[i];
+        myj[h] = b.lane_ch[64 * W + i];
+    }
     for (int c = gwave(); c < b.n_chain; c += nwaves()) {
         const int L = b.ch_L[c], R = b.ch_R[c], off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
         const double *pre = b.ch_pre + off;
         const ChainPfx P{pre, 0, 1};
-        int aL = 0, aR = 0;
-        uint64_t wL = 0, wR = 0;
-        double sL = 1.0, sR = 1.0, tL = 0.0, tR = 0.0;
-        if (act) {
-            aL = b.ep_lvl[(int64_t)b.ep_idx[L] * 64 + lane];
-            aR = b.ep_lvl[(int64_t)b.ep_idx[R] * 64 + lane];
-            wL = sg<SM>(b, (int64_t)L * 64 + lane);
-            wR = sg<SM>(b, (int64_t)R * 64 + lane);
-            sL = sg_val<SM>(wL);
-            sR = sg_val<SM>(wR);
-            if (c == myc) {
-                tL = (double)tt[(int64_t)L * 64 + lane];
-                tR = (double)tt[(int64_t)R * 64 + lane];
+        const int64_t eL = (int64_t)b.ep_idx[L] * 64, eR = (int64_t)b.ep_idx[R] * 64;
+        int aL[W], aR[W], t[W], mA[W], b0[W];
+        bool tie[W], own[W];
+        uint64_t wL[W], wR[W], wT[W];
+        double sL[W], sR[W], tL[W], tR[W], cA[W], cB[W];
+        bool any_own = false;
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+            aL[h] = aR[h] = 0;
+            wL[h] = wR[h] = 0;
+            sL[h] = sR[h] = 1.0;
+            tL[h] = tR[h] = 0.0;
+            if (act[h]) {
+                aL[h] = b.ep_lvl[(eL + lane) * W + h];
+                aR[h] = b.ep_lvl[(eR + lane) * W + h];
+                wL[h] = sg<SM>(b, ((int64_t)L * 64 + lane) * W + h);
+                wR[h] = sg<SM>(b, ((int64_t)R * 64 + lane) * W + h);
+                sL[h] = sg_val<SM>(wL[h]);
+                sR[h] = sg_val<SM>(wR[h]);
+                if (c == myc[h]) {
+                    tL[h] = (double)tt[((int64_t)L * 64 + lane) * W + h];
+                    tR[h] = (double)tt[((int64_t)R * 64 + lane) * W + h];
+                }
             }
+            own[h] = act[h] && c == myc[h];
+            any_own |= __ballot(own[h]) != 0;
+            // segment L .. R seen from L: split, tie and the prefix sums there
+            const int S = aR[h] - aL[h] + K;
+            t[h] = -1;
+            tie[h] = false;
+            if ((S & 1) == 0) {
+                t[h] = S >> 1;
+                mA[h] = t[h] - 1;
+                tie[h] = t[h] >= 1 && t[h] <= K - 1;
+                b0[h] = t[h] + 1;
+            } else {
+                mA[h] = (S - 1) >> 1;
+                b0[h] = mA[h] + 1;
+            }
+            mA[h] = min(mA[h], K - 1);
+            const double tw = tie[h] ? P(t[h]) - P(t[h] - 1) : 0.0;
+            cA[h] = P(mA[h]) + (tie[h] ? tw * sL[h] / (sL[h] + sR[h]) : 0.0);
+            cB[h] = P(min(b0[h], K) - 1) - (tie[h] ? tw * sR[h] / (sL[h] + sR[h]) : 0.0);
+            wT[h] = F64 ? d2u(sL[h] + sR[h]) : wL[h] + wR[h];
         }
-        const bool own = act && c == myc;
-        const bool any_own = __ballot(own) != 0;
-        // segment L .. R seen from L: split, tie and the prefix sums there
-        const int S = aR - aL + K;
-        int t = -1, mA, b0;
-        bool tie = false;
-        if ((S & 1) == 0) {
-            t = S >> 1;
-            mA = t - 1;
-            tie = t >= 1 && t <= K - 1;
-            b0 = t + 1;
-        } else {
-            mA = (S - 1) >> 1;
-            b0 = mA + 1;
-        }
-        mA = min(mA, K - 1);
-        const double tw = tie ? P(t) - P(t - 1) : 0.0;
-        const double cA = P(mA) + (tie ? tw * sL / (sL + sR) : 0.0);
-        const double cB = P(min(b0, K) - 1) - (tie ? tw * sR / (sL + sR) : 0.0);
-        const uint64_t wT = F64 ? d2u(sL + sR) : wL + wR;
         double p0 = 0.0;  // P(i - 1)
         // the interiors' BC words, 64 at a time, one per lane (coalesced)
-        double acc = 0.0;
+        double accbc = 0.0;
         for (int i = 1; i < K; ++i) {
             const int64_t x = (int64_t)b.n + off + i - 1;
-            if (((i - 1) & 63) == 0 && i - 1 + lane < K - 1) acc = b.bc[x + lane];
+            if (((i - 1) & 63) == 0 && i - 1 + lane < K - 1) accbc = b.bc[x + lane];
             const double p1 = pre[i - 1];  // P(i)
-            int lev;
-            uint64_t sw;
-            double dl;
-            if (i <= mA) {
-                lev = aL + i;
-                sw = wL;
-                dl = cA - p1;
-            } else if (tie && i == t) {
-                lev = aL + t;
-                sw = wT;
-                dl = 0.0;
-            } else {
-                lev = aR + K - i;
-                sw = wR;
-                dl = p0 - cB;
-            }
-            if (any_own && own) {
-                lev = 0;
-                sw = one;
-                dl = 0.0;
-                if (i < myj)  // the half from the source to L, numbered from the source
-                    seg_interior<SM>(0, aL, 1.0, sL, one, wL, myj, myj - i, ChainPfx{pre, myj - 1, -1}, tL, lev, sw,
-                                     dl);
-                else if (i > myj)  // the half from the source to R
-                    seg_interior<SM>(0, aR, 1.0, sR, one, wR, K - myj, i - myj, ChainPfx{pre, myj, 1}, tR, lev, sw,
-                                     dl);
+            double contrib = 0.0;
+#pragma unroll
+            for (int h = 0; h < W; ++h) {
+                int lev;
+                uint64_t sw;
+                double dl;
+                if (i <= mA[h]) {
+                    lev = aL[h] + i;
+                    sw = wL[h];
+                    dl = cA[h] - p1;
+                } else if (tie[h] && i == t[h]) {
+                    lev = aL[h] + t[h];
+                    sw = wT[h];
+                    dl = 0.0;
+                } else {
+                    lev = aR[h] + K - i;
+                    sw = wR[h];
+                    dl = p0 - cB[h];
+                }
+                if (any_own && own[h]) {
+                    lev = 0;
+                    sw = one;
+                    dl = 0.0;
+                    const int j = myj[h];
+                    if (i < j)  // the half from the source to L, numbered from the source
+                        seg_interior<SM>(0, aL[h], 1.0, sL[h], one, wL[h], j, j - i, ChainPfx{pre, j - 1, -1},
+                                         tL[h], lev, sw, dl);
+                    else if (i > j)  // the half from the source to R
+                        seg_interior<SM>(0, aR[h], 1.0, sR[h], one, wR[h], K - j, i - j, ChainPfx{pre, j, 1},
+                                         tR[h], lev, sw, dl);
+                }
+                if (act[h]) {
+                    contrib += wgt[h] * dl;
+                    if (b.lvl) {
+                        const int64_t cell = (x * 64 + lane) * W + h;
+                        b.lvl[cell] = lev;
+                        sg_put<SM>(b, cell, sw);
+                        b.dbg_delta[cell] = dl;
+                    }
+                }
             }
             p0 = p1;
-            const double contrib = wave_sum(act ? w * dl : 0.0);
-            if (lane == ((i - 1) & 63)) acc += contrib;
+            contrib = wave_sum(contrib);
+            if (lane == ((i - 1) & 63)) accbc += contrib;
             if ((((i - 1) & 63) == 63 || i == K - 1) && lane <= ((i - 1) & 63))
-                b.bc[(int64_t)b.n + off + ((i - 1) & ~63) + lane] = acc;
-            if (b.lvl && act) {
-                b.lvl[x * 64 + lane] = lev;
-                sg_put<SM>(b, x * 64 + lane, sw);
-                b.dbg_delta[x * 64 + lane] = dl;
-            }
+                b.bc[(int64_t)b.n + off + ((i - 1) & ~63) + lane] = accbc;
         }
     }
 }
 
 // After the backward only Q_1 (mask array 1) and Q_2 (array 2) are set.
+template <int W>
 __global__ __launch_bounds__(BB) void k_b_finish(BatchDev b) {
     const int D = b.ctr[3];
     if (D < 0 || b.ctr[2]) return;
     const int a0 = b.qoff[1], a1 = D >= 1 ? b.qoff[2] : a0;
     const int c0 = D >= 2 ? b.qoff[2] : 0, c1 = D >= 2 ? b.qoff[3] : 0;
-    for (int i = a0 + blockIdx.x * BB + threadIdx.x; i < a1; i += gridDim.x * BB) b.mb1[b.Q[i]] = 0;
-    for (int i = c0 + blockIdx.x * BB + threadIdx.x; i < c1; i += gridDim.x * BB) b.mb2[b.Q[i]] = 0;
+    for (int i = a0 + blockIdx.x * BB + threadIdx.x; i < a1; i += gridDim.x * BB) mst<W>(b.mb1, b.Q[i], mzero<W>());
+    for (int i = c0 + blockIdx.x * BB + threadIdx.x; i < c1; i += gridDim.x * BB) mst<W>(b.mb2, b.Q[i], mzero<W>());
 }
 
 // Clears every mask entry of Q[0, end) (after an abandoned forward).
+template <int W>
 __global__ __launch_bounds__(BB) void k_b_clear_all(BatchDev b, int end) {
     for (int i = blockIdx.x * BB + threadIdx.x; i < end; i += gridDim.x * BB) {
         const int v = b.Q[i];
-        b.mb0[v] = 0;
-        b.mb1[v] = 0;
-        b.mb2[v] = 0;
+        mst<W>(b.mb0, v, mzero<W>());
+        mst<W>(b.mb1, v, mzero<W>());
+        mst<W>(b.mb2, v, mzero<W>());
     }
 }
 
@@ -950,40 +1203,56 @@ __global__ __launch_bounds__(BB) void k_b_fold(double *dst, double *src, const i
 }
 
 // ------------------------------------------------------------ launchers ---
+template <int W>
+static void init_w(const BatchDev &b, int ns, int sm, int g, hipStream_t st) {
+    hipLaunchKernelGGL(k_b_clear_vis<W>, dim3(std::max(g, 1)), dim3(BB), 0, st, b, sm);
+    if (sm == SIG_U32) hipLaunchKernelGGL((k_b_seed<SIG_U32, W>), dim3(1), dim3(64), 0, st, b, ns);
+    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_seed<SIG_U64, W>), dim3(1), dim3(64), 0, st, b, ns);
+    else hipLaunchKernelGGL((k_b_seed<SIG_F64, W>), dim3(1), dim3(64), 0, st, b, ns);
+}
+
 hipError_t launch_b_init(const BatchDev &b, int ns, int sm, int cap, hipStream_t st) {
-    int g = (int)std::min<int64_t>(((int64_t)b.n + BB - 1) / BB, cap);
-    hipLaunchKernelGGL(k_b_clear_vis, dim3(std::max(g, 1)), dim3(BB), 0, st, b, sm);
-    if (sm == SIG_U32) hipLaunchKernelGGL(k_b_seed<SIG_U32>, dim3(1), dim3(64), 0, st, b, ns);
-    else if (sm == SIG_U64) hipLaunchKernelGGL(k_b_seed<SIG_U64>, dim3(1), dim3(64), 0, st, b, ns);
-    else hipLaunchKernelGGL(k_b_seed<SIG_F64>, dim3(1), dim3(64), 0, st, b, ns);
+    const int g = (int)std::min<int64_t>(((int64_t)b.n + BB - 1) / BB, cap);
+    if (b.spl == 2) init_w<2>(b, ns, sm, g, st);
+    else init_w<1>(b, ns, sm, g, st);
     return hipGetLastError();
+}
+
+template <int W>
+static void forward_w(const BatchDev &b, int sm, int l, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_b_expand<W>, dim3(grid), dim3(BB), 0, st, b, l);
+    if (sm == SIG_U32) hipLaunchKernelGGL((k_b_pull<SIG_U32, W>), dim3(grid), dim3(BB), 0, st, b, l);
+    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_pull<SIG_U64, W>), dim3(grid), dim3(BB), 0, st, b, l);
+    else hipLaunchKernelGGL((k_b_pull<SIG_F64, W>), dim3(grid), dim3(BB), 0, st, b, l);
 }
 
 hipError_t launch_b_forward(const BatchDev &b, int sm, int l, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_b_expand, dim3(grid), dim3(BB), 0, st, b, l);
-    if (sm == SIG_U32) hipLaunchKernelGGL(k_b_pull<SIG_U32>, dim3(grid), dim3(BB), 0, st, b, l);
-    else if (sm == SIG_U64) hipLaunchKernelGGL(k_b_pull<SIG_U64>, dim3(grid), dim3(BB), 0, st, b, l);
-    else hipLaunchKernelGGL(k_b_pull<SIG_F64>, dim3(grid), dim3(BB), 0, st, b, l);
+    if (b.spl == 2) forward_w<2>(b, sm, l, grid, st);
+    else forward_w<1>(b, sm, l, grid, st);
     return hipGetLastError();
-}
-
-template <int SM>
-static void back_tt(const BatchDev &b, bool fp32, int l, int grid, hipStream_t st) {
-    if (fp32) hipLaunchKernelGGL((k_b_back<SM, float>), dim3(grid), dim3(BB), 0, st, b, l);
-    else hipLaunchKernelGGL((k_b_back<SM, double>), dim3(grid), dim3(BB), 0, st, b, l);
 }
 
 // FP64-sigma tiles keep FP64 terms in either precision: 1/sigma can leave
 // the f32 range.
+template <int W>
+static void back_w(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st) {
+    if (sm == SIG_U32 && fp32) hipLaunchKernelGGL((k_b_back<SIG_U32, float, W>), dim3(grid), dim3(BB), 0, st, b, l);
+    else if (sm == SIG_U32) hipLaunchKernelGGL((k_b_back<SIG_U32, double, W>), dim3(grid), dim3(BB), 0, st, b, l);
+    else if (sm == SIG_U64 && fp32)
+        hipLaunchKernelGGL((k_b_back<SIG_U64, float, W>), dim3(grid), dim3(BB), 0, st, b, l);
+    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_back<SIG_U64, double, W>), dim3(grid), dim3(BB), 0, st, b, l);
+    else hipLaunchKernelGGL((k_b_back<SIG_F64, double, W>), dim3(grid), dim3(BB), 0, st, b, l);
+}
+
 hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st) {
-    if (sm == SIG_U32) back_tt<SIG_U32>(b, fp32, l, grid, st);
-    else if (sm == SIG_U64) back_tt<SIG_U64>(b, fp32, l, grid, st);
-    else hipLaunchKernelGGL((k_b_back<SIG_F64, double>), dim3(grid), dim3(BB), 0, st, b, l);
+    if (b.spl == 2) back_w<2>(b, sm, fp32, l, grid, st);
+    else back_w<1>(b, sm, fp32, l, grid, st);
     return hipGetLastError();
 }
 
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_b_finish, dim3(grid), dim3(BB), 0, st, b);
+    if (b.spl == 2) hipLaunchKernelGGL(k_b_finish<2>, dim3(grid), dim3(BB), 0, st, b);
+    else hipLaunchKernelGGL(k_b_finish<1>, dim3(grid), dim3(BB), 0, st, b);
     return hipGetLastError();
 }
 
@@ -992,22 +1261,32 @@ static int chain_grid(const BatchDev &b) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)b.n_chain + NW - 1) / NW, 1 << 16));
 }
 
+template <int W>
+static void chain_check_w(const BatchDev &b, int sm, int g, hipStream_t st) {
+    if (sm == SIG_U32) hipLaunchKernelGGL((k_b_chain_check<SIG_U32, W>), dim3(g), dim3(BB), 0, st, b);
+    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_chain_check<SIG_U64, W>), dim3(g), dim3(BB), 0, st, b);
+}
+
 hipError_t launch_b_chain_check(const BatchDev &b, int sm, int grid, hipStream_t st) {
     (void)grid;
-    grid = chain_grid(b);
-    if (sm == SIG_U32) hipLaunchKernelGGL(k_b_chain_check<SIG_U32>, dim3(grid), dim3(BB), 0, st, b);
-    else if (sm == SIG_U64) hipLaunchKernelGGL(k_b_chain_check<SIG_U64>, dim3(grid), dim3(BB), 0, st, b);
+    if (b.spl == 2) chain_check_w<2>(b, sm, chain_grid(b), st);
+    else chain_check_w<1>(b, sm, chain_grid(b), st);
     return hipGetLastError();
+}
+
+template <int W>
+static void chains_w(const BatchDev &b, int sm, bool fp32, int g, hipStream_t st) {
+    if (sm == SIG_U32 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U32, float, W>), dim3(g), dim3(BB), 0, st, b);
+    else if (sm == SIG_U32) hipLaunchKernelGGL((k_b_chains<SIG_U32, double, W>), dim3(g), dim3(BB), 0, st, b);
+    else if (sm == SIG_U64 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U64, float, W>), dim3(g), dim3(BB), 0, st, b);
+    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_chains<SIG_U64, double, W>), dim3(g), dim3(BB), 0, st, b);
+    else hipLaunchKernelGGL((k_b_chains<SIG_F64, double, W>), dim3(g), dim3(BB), 0, st, b);
 }
 
 hipError_t launch_b_chains(const BatchDev &b, int sm, bool fp32, int grid, hipStream_t st) {
     (void)grid;
-    grid = chain_grid(b);
-    if (sm == SIG_U32 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U32, float>), dim3(grid), dim3(BB), 0, st, b);
-    else if (sm == SIG_U32) hipLaunchKernelGGL((k_b_chains<SIG_U32, double>), dim3(grid), dim3(BB), 0, st, b);
-    else if (sm == SIG_U64 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U64, float>), dim3(grid), dim3(BB), 0, st, b);
-    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_chains<SIG_U64, double>), dim3(grid), dim3(BB), 0, st, b);
-    else hipLaunchKernelGGL((k_b_chains<SIG_F64, double>), dim3(grid), dim3(BB), 0, st, b);
+    if (b.spl == 2) chains_w<2>(b, sm, fp32, chain_grid(b), st);
+    else chains_w<1>(b, sm, fp32, chain_grid(b), st);
     return hipGetLastError();
 }
 
@@ -1018,7 +1297,8 @@ hipError_t launch_b_fold(double *dst, double *src, const int32_t *perm, int n, i
 
 hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st) {
     if (end <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_b_clear_all, dim3(grid), dim3(BB), 0, st, b, end);
+    if (b.spl == 2) hipLaunchKernelGGL(k_b_clear_all<2>, dim3(grid), dim3(BB), 0, st, b, end);
+    else hipLaunchKernelGGL(k_b_clear_all<1>, dim3(grid), dim3(BB), 0, st, b, end);
     return hipGetLastError();
 }
 

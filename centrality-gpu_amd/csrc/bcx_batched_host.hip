@@ -89,9 +89,11 @@ static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
     // ensure has released its buffer, so the next call must allocate again
     if (n > s.n_cap) {
         s.n_cap = 0;
-        if ((rc = s.vis.ensure(ctx, 8 * N, "batched vis")) || (rc = s.mb.ensure(ctx, 24 * N, "batched masks")) ||
-            (rc = s.sig.ensure(ctx, 8 * 64 * N, "batched sigma")) ||
-            (rc = s.tt.ensure(ctx, 8 * 64 * N, "batched dependency terms")) ||
+        // (every per-vertex and per-entry array holds MAX_SPL sources per lane)
+        if ((rc = s.vis.ensure(ctx, 8 * MAX_SPL * N, "batched vis")) ||
+            (rc = s.mb.ensure(ctx, 24 * MAX_SPL * N, "batched masks")) ||
+            (rc = s.sig.ensure(ctx, 8 * 64 * MAX_SPL * N, "batched sigma")) ||
+            (rc = s.tt.ensure(ctx, 8 * 64 * MAX_SPL * N, "batched dependency terms")) ||
             (rc = s.qoff.ensure(ctx, 4 * (N + 8), "batched level offsets")) ||
             (rc = s.bcacc.ensure(ctx, 8 * N, "batched BC accumulator")))
             return rc;
@@ -104,14 +106,15 @@ static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
         const int64_t q = std::min<int64_t>((int64_t)8 * n + 4096, INT32_MAX / 2);
         s.qcap = 0;
         if ((rc = s.Q.ensure(ctx, 4 * (size_t)q, "batched lists")) ||
-            (rc = s.Qm.ensure(ctx, 8 * (size_t)q, "batched lists")))
+            (rc = s.Qm.ensure(ctx, 8 * MAX_SPL * (size_t)q, "batched lists")))
             return rc;
         s.qcap = (int32_t)q;
     }
-    if ((rc = s.ctr.ensure(ctx, 64, "batched counters")) || (rc = s.wsl.ensure(ctx, 8 * BATCH_W, "tile weights")))
+    if ((rc = s.ctr.ensure(ctx, 64, "batched counters")) ||
+        (rc = s.wsl.ensure(ctx, 8 * 64 * MAX_SPL, "tile weights")))
         return rc;
-    if (probe && ((rc = s.lvl.ensure(ctx, 4 * 64 * N, "probe levels")) ||
-                  (rc = s.dbg.ensure(ctx, 8 * 64 * N, "probe delta"))))
+    if (probe && ((rc = s.lvl.ensure(ctx, 4 * 64 * MAX_SPL * N, "probe levels")) ||
+                  (rc = s.dbg.ensure(ctx, 8 * 64 * MAX_SPL * N, "probe delta"))))
         return rc;
     return BCX_OK;
 }
@@ -120,7 +123,7 @@ static int slot_ensure(bcx_ctx *ctx, Slot &s, int32_t n, bool probe) {
 static int bs_ensure(bcx_ctx *ctx, int32_t n, int32_t n_src, int want_slots, bool probe) {
     if (!ctx->batched) ctx->batched = new BatchedState();
     BatchedState &bs = *ctx->batched;
-    const size_t per = (size_t)n * (8 + 24 + 512 + 512 + 8 * 12) + (1 << 20);
+    const size_t per = (size_t)n * MAX_SPL * (8 + 24 + 512 + 512 + 8 * 12) + (1 << 20);
     const size_t room = ctx->mem_cap > ctx->mem_used ? (ctx->mem_cap - ctx->mem_used) : 0;
     int fit = (int)std::max<size_t>(1, std::min<size_t>(MAX_SLOTS, room / 2 / std::max<size_t>(per, 1)));
     bs.n_slots = std::max(1, std::min(want_slots, fit));
@@ -158,6 +161,7 @@ struct TileJob {
     uint64_t limit;
     bool fp32;
     int grid;
+    int spl;      // sources per lane (W): tiles of 64 W sources
     bool graphs;  // replay captured level launches (first chunk of a tile)
 };
 
@@ -173,8 +177,8 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const TileJob &j, uint64_t limit
     b.reach = dc.dev.reach;
     b.vis = (uint64_t *)s.vis.p;
     b.mb0 = (uint64_t *)s.mb.p;
-    b.mb1 = b.mb0 + s.n_cap;
-    b.mb2 = b.mb1 + s.n_cap;
+    b.mb1 = b.mb0 + (size_t)s.n_cap * MAX_SPL;
+    b.mb2 = b.mb1 + (size_t)s.n_cap * MAX_SPL;
     b.sig = s.sig.p;
     b.tt = s.tt.p;
     b.lvl = bs.probe ? (int32_t *)s.lvl.p : nullptr;
@@ -196,8 +200,9 @@ static BatchDev make_dev(bcx_ctx *ctx, Slot &s, const TileJob &j, uint64_t limit
     b.hub_mask = (uint64_t *)s.hub_mask.p;
     b.hub_acc = (double *)s.hub_acc.p;
     b.hub_cnt = (int32_t *)s.hub_cnt.p;
-    b.srcs = (const int32_t *)bs.srcs.p + (size_t)s.tile * BATCH_W;
-    b.ws = (const double *)bs.ws.p + (size_t)s.tile * BATCH_W;
+    b.spl = j.spl;
+    b.srcs = (const int32_t *)bs.srcs.p + (size_t)s.tile * BATCH_W * j.spl;
+    b.ws = (const double *)bs.ws.p + (size_t)s.tile * BATCH_W * j.spl;
     b.wsl = (double *)s.wsl.p;
     b.bc = (double *)s.bcacc.p;
     b.limit = limit;
@@ -318,8 +323,9 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
     if (++s.attempts > 6) { *rc_out = ctx_fail(ctx, BCX_E_OVERFLOW, "batched tile could not be completed"); return 0; }
     if (h[2]) {
         // lists full: masks may hold entries of vertices never listed
-        const int64_t q2 = std::min<int64_t>((int64_t)s.qcap * 4, std::min<int64_t>((int64_t)64 * j.dc->n + 4096,
-                                                                                     INT32_MAX / 2));
+        // (a vertex is an entry at most once per source of the tile)
+        const int64_t q2 = std::min<int64_t>((int64_t)s.qcap * 4,
+                                             std::min<int64_t>((int64_t)64 * MAX_SPL * j.dc->n + 4096, INT32_MAX / 2));
         if (q2 <= s.qcap) { *rc_out = ctx_fail(ctx, BCX_E_NOMEM, "batched level lists cannot grow"); return 0; }
         if ((rc = ctx_hip(ctx, hipMemsetAsync(s.mb.p, 0, s.mb.bytes, s.stream), "memset")) ||
             (rc = ctx_hip(ctx, hipStreamSynchronize(s.stream), "sync"))) {
@@ -328,7 +334,7 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
         }
         s.qcap = 0;  // (a failed ensure has released the list it was growing)
         if ((rc = s.Q.ensure(ctx, 4 * (size_t)q2, "batched lists (growth)")) ||
-            (rc = s.Qm.ensure(ctx, 8 * (size_t)q2, "batched lists (growth)"))) {
+            (rc = s.Qm.ensure(ctx, 8 * MAX_SPL * (size_t)q2, "batched lists (growth)"))) {
             *rc_out = rc;
             return 0;
         }
@@ -364,7 +370,8 @@ static int poll_tile(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st, int
 static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob &j, bcx_stats *st,
                      int *sm_out) {
     BatchedState &bs = *ctx->batched;
-    const int64_t ntiles = (n_src + BATCH_W - 1) / BATCH_W;
+    const int64_t TW = (int64_t)BATCH_W * j.spl;  // sources per tile
+    const int64_t ntiles = (n_src + TW - 1) / TW;
     const int ns_slots = (int)std::min<int64_t>(bs.n_slots, ntiles);
     int rc;
     hipEvent_t e0;
@@ -384,7 +391,7 @@ static int run_tiles(bcx_ctx *ctx, const DComp &dc, int64_t n_src, const TileJob
             if (s.tile < 0) {
                 if (next >= ntiles) continue;
                 s.attempts = 0;
-                const int ns = (int)std::min<int64_t>(BATCH_W, n_src - next * BATCH_W);
+                const int ns = (int)std::min<int64_t>(TW, n_src - next * TW);
                 // start in the mode the last tile needed (neighbouring tiles
                 // have similar counts); every 16th tile tries 32-bit words again
                 const int sm = (next % 16 == 0) ? SIG_U32 : bs.last_sm;
@@ -460,18 +467,19 @@ static int stage_hubs(bcx_ctx *ctx, const DComp &dc) {
         return rc;
     for (int i = 0; i < bs.n_slots; ++i) {
         Slot &s = bs.slots[i];
-        if ((rc = s.hub_mask.ensure(ctx, 8 * (size_t)nh * bs.hub_stride, "hub masks")) ||
-            (rc = s.hub_acc.ensure(ctx, 8 * 64 * (size_t)nh, "hub accumulators")) ||
+        if ((rc = s.hub_mask.ensure(ctx, 8 * MAX_SPL * (size_t)nh * bs.hub_stride, "hub masks")) ||
+            (rc = s.hub_acc.ensure(ctx, 8 * 64 * MAX_SPL * (size_t)nh, "hub accumulators")) ||
             (rc = s.hub_cnt.ensure(ctx, 4 * (size_t)nh, "hub counters")))
             return rc;
     }
     return ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "hubs");
 }
 
-static int stage_sources(bcx_ctx *ctx, const DComp &dc, const std::vector<int32_t> &srcs) {
-    const size_t ntiles = (srcs.size() + BATCH_W - 1) / BATCH_W;
-    std::vector<int32_t> hs(ntiles * BATCH_W, -1);
-    std::vector<double> hw(ntiles * BATCH_W, 0.0);
+// Sources and weights of tiles of tw sources (the last one padded).
+static int stage_sources(bcx_ctx *ctx, const DComp &dc, const std::vector<int32_t> &srcs, int tw) {
+    const size_t ntiles = (srcs.size() + tw - 1) / tw;
+    std::vector<int32_t> hs(ntiles * tw, -1);
+    std::vector<double> hw(ntiles * tw, 0.0);
     for (size_t i = 0; i < srcs.size(); ++i) {
         hs[i] = srcs[i];
         hw[i] = (double)dc.host.reach[srcs[i]] + 1.0;
@@ -561,6 +569,7 @@ static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double
     j.fp32 = o->precision == BCX_PREC_FP32C;
     j.grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)dc.n + 3) / 4, (int64_t)ctx->n_cu * bpc));
     j.graphs = !(o->flags & BCX_FLAG_NO_GRAPH);
+    j.spl = 1;
     return j;
 }
 
@@ -1103,7 +1112,7 @@ static void set_chains(BatchDev &b, Slot &s, const ChainCut *cut) {
     b.ep_list = cut->ep_list;
     b.ep_lvl = (int32_t *)s.eplv.p;
     b.lane_ch = (int32_t *)s.lanes.p;
-    b.lane_mask = (uint64_t *)((char *)s.lanes.p + 2 * 64 * sizeof(int32_t));
+    b.lane_mask = (uint64_t *)((char *)s.lanes.p + 2 * 64 * MAX_SPL * sizeof(int32_t));
 }
 
 // Per-slot buffers of a run with contracted chains.
@@ -1112,8 +1121,8 @@ static int cut_slots(bcx_ctx *ctx, const ChainCut *cut) {
     int rc;
     for (int i = 0; i < bs.n_slots; ++i) {
         Slot &s = bs.slots[i];
-        if ((rc = s.eplv.ensure(ctx, 4 * 64 * (size_t)std::max(cut->n_ep, 1), "endpoint levels")) ||
-            (rc = s.lanes.ensure(ctx, 2 * 64 * sizeof(int32_t) + sizeof(uint64_t), "lane chains")))
+        if ((rc = s.eplv.ensure(ctx, 4 * 64 * MAX_SPL * (size_t)std::max(cut->n_ep, 1), "endpoint levels")) ||
+            (rc = s.lanes.ensure(ctx, (2 * 64 * sizeof(int32_t) + sizeof(uint64_t)) * MAX_SPL, "lane chains")))
             return rc;
     }
     return BCX_OK;
@@ -1157,11 +1166,18 @@ static int tile_graph(bcx_ctx *ctx, DComp &dc, bool want, DComp *&pc, ChainCut *
 struct MultiSched {
     bool cut;
     int bpc, slots;
+    int spl;  // sources per lane
 };
+
+// sources per lane unless tuned or fixed: two (128-source tiles) measured
+// above one on the Chung-Lu and R-MAT cores (fewer entries and arc walks per
+// source: a tile reaches a vertex at few more distinct levels)
+constexpr int DEFAULT_SPL = 2;
 
 static MultiSched multi_sched(const DComp &dc, const bcx_opts *o) {
     MultiSched m;
     m.cut = want_cut(o) && dc.sched_cut != 0;
+    m.spl = (o->flags & BCX_FLAG_TILE64) ? 1 : (dc.sched_spl > 0 ? dc.sched_spl : DEFAULT_SPL);
     m.bpc = o->blocks_per_cu > 0 ? std::min(o->blocks_per_cu, 64) : (dc.sched_bpc > 0 ? dc.sched_bpc : DEFAULT_BPC);
     m.slots = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width))
                                  : (dc.sched_slots > 0 ? dc.sched_slots : MAX_SLOTS);
@@ -1178,14 +1194,16 @@ static int multi_run(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs, 
     if (rc) return rc;
     std::vector<int32_t> ps(srcs.size());
     for (size_t i = 0; i < srcs.size(); ++i) ps[i] = (*inv)[srcs[i]];
-    const int64_t ntiles = ((int64_t)ps.size() + BATCH_W - 1) / BATCH_W;
-    rc = bs_ensure(ctx, pc->n, (int32_t)(ntiles * BATCH_W), m.slots, false);
+    const int tw = BATCH_W * m.spl;
+    const int64_t ntiles = ((int64_t)ps.size() + tw - 1) / tw;
+    rc = bs_ensure(ctx, pc->n, (int32_t)(ntiles * tw), m.slots, false);
     if (rc) return rc;
     if (cut && (rc = cut_slots(ctx, cut))) return rc;
-    if ((rc = stage_sources(ctx, *pc, ps)) || (rc = stage_hubs(ctx, *pc))) return rc;
+    if ((rc = stage_sources(ctx, *pc, ps, tw)) || (rc = stage_hubs(ctx, *pc))) return rc;
     TileJob j = make_job(ctx, *pc, o, bc, m.bpc);
     j.perm = perm;
     j.cut = cut;
+    j.spl = m.spl;
     if (st && cut) st->n_contracted += pc->n - cut->n_core;
     return run_tiles(ctx, *pc, (int64_t)ps.size(), j, st, nullptr);
 }
@@ -1575,9 +1593,10 @@ int batched_schedule(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, bcx_stats *st) 
     if (!rc && tune_multi) {
         std::vector<MultiSched> cands;
         if (o->blocks_per_cu <= 0)
-            for (int bpc : {4, 16}) cands.push_back({base.cut, bpc, base.slots});
-        if (o->batch_width <= 0) cands.push_back({base.cut, base.bpc, 4});
-        if (base.cut) cands.push_back({false, base.bpc, base.slots});
+            for (int bpc : {4, 16}) cands.push_back({base.cut, bpc, base.slots, base.spl});
+        if (o->batch_width <= 0) cands.push_back({base.cut, base.bpc, 4, base.spl});
+        if (base.cut) cands.push_back({false, base.bpc, base.slots, base.spl});
+        if (!(o->flags & BCX_FLAG_TILE64)) cands.push_back({base.cut, base.bpc, base.slots, 3 - base.spl});
         double r_best = r_multi;
         for (const MultiSched &m : cands) {
             double r = 0.0;
@@ -1598,6 +1617,7 @@ int batched_schedule(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, bcx_stats *st) 
         dc.sched_multi = true;
         dc.sched_cut = (base.cut && !best.cut) ? 0 : 1;
         dc.sched_bpc = (int16_t)(o->blocks_per_cu > 0 ? 0 : best.bpc);
+        dc.sched_spl = (int8_t)((o->flags & BCX_FLAG_TILE64) ? 0 : best.spl);
         dc.sched_slots = (int16_t)(o->batch_width > 0 ? 0 : best.slots);
     }
     if (st) st->n_tuned++;
@@ -1616,7 +1636,14 @@ int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *s
     DComp &dc = g->comps[0];
     const int32_t n = dc.n;
     const int kind = batched_kind(dc, o->flags);
-    const size_t cells = (size_t)n * BATCH_W;
+    // sources per lane of the tile: the multi-launch tiles' schedule (the
+    // other kernels run 64-source tiles)
+    const int spl = kind == BCX_KERNEL_MULTI ? multi_sched(dc, o).spl : 1;
+    if (ns > BATCH_W * spl) {
+        bcx_dgraph_free(ctx, g);
+        return ctx_fail(ctx, BCX_E_ARG, "debug tile: more sources than the kernel's tile holds");
+    }
+    const size_t cells = (size_t)n * BATCH_W * spl;
     std::vector<int32_t> lv(cells, 0), at((size_t)n);  // at[v]: row of v in the device arrays
     std::vector<uint64_t> sg(cells, 0);
     std::vector<double> dl(cells, 0.0);
@@ -1665,14 +1692,15 @@ int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *s
             std::vector<int32_t> ps((size_t)ns);
             for (int32_t i = 0; i < ns; ++i) ps[(size_t)i] = (*inv)[srcs[i]];
             for (int32_t v = 0; v < n; ++v) at[v] = (*inv)[v];
-            rc = bs_ensure(ctx, n, BATCH_W, 1, true);
+            rc = bs_ensure(ctx, n, BATCH_W * spl, 1, true);
             if (!rc && cut) rc = cut_slots(ctx, cut);
-            if (!rc) rc = stage_sources(ctx, *pc, ps);
+            if (!rc) rc = stage_sources(ctx, *pc, ps, BATCH_W * spl);
             if (!rc) rc = stage_hubs(ctx, *pc);
             if (!rc) {
                 TileJob j = make_job(ctx, *pc, o, bc);
                 j.perm = perm;
                 j.cut = cut;
+                j.spl = spl;
                 rc = run_tiles(ctx, *pc, ns, j, nullptr, &sm);
             }
         }
@@ -1698,9 +1726,11 @@ int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *s
     (void)ctx->timer.collect();
     bcx_dgraph_free(ctx, g);
     if (rc) return rc;
+    // source i is word h = i / 64 of lane k = i % 64: cell (v * 64 + k) * spl + h
     for (int32_t i = 0; i < ns; ++i)
         for (int32_t v = 0; v < n; ++v) {
-            const size_t c = (size_t)at[v] * BATCH_W + (size_t)i, r = (size_t)i * (size_t)n + (size_t)v;
+            const size_t c = ((size_t)at[v] * BATCH_W + (size_t)(i % BATCH_W)) * spl + (size_t)(i / BATCH_W);
+            const size_t r = (size_t)i * (size_t)n + (size_t)v;
             double f;
             std::memcpy(&f, &sg[c], 8);
             if (level) level[r] = lv[c];

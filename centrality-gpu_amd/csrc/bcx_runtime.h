@@ -80,6 +80,7 @@ struct DComp {
     int8_t sched_cut = 1;     // multi-launch: 1 contracted chains, 0 the relabelled copy
     int16_t sched_bpc = 0;    // multi-launch: level-kernel blocks per CU (0: default)
     int16_t sched_slots = 0;  // multi-launch: tiles in flight (0: default)
+    int8_t sched_spl = 0;     // multi-launch: sources per lane, 1 or 2 (0: default)
 };
 
 // One endpoint-row batch: BFS work (endpoints, with slot and mode) then the

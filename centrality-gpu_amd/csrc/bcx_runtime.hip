@@ -822,7 +822,7 @@ int bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src, const
 int bcx_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns, const bcx_opts *opts,
                    int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta, int32_t *sigma_is_f64,
                    int32_t *kernel_used) {
-    if (!ctx || !comp || !srcs || ns < 1 || ns > BATCH_W || comp->n <= 0)
+    if (!ctx || !comp || !srcs || ns < 1 || ns > BATCH_W * MAX_SPL || comp->n <= 0)
         return ctx_fail(ctx, BCX_E_ARG, "bad debug tile args");
     std::vector<uint8_t> seen((size_t)comp->n, 0);
     for (int32_t i = 0; i < ns; ++i) {

@@ -193,6 +193,10 @@ typedef struct {
  * per CU, 8 tiles, contraction on).  Options that fix a choice (the kernel
  * flags, batch_width, blocks_per_cu, NO_CONTRACT) are kept and not tuned.  */
 #define BCX_FLAG_NO_TUNE         32
+/* Batched path, multi-launch tiles: 64 sources per tile (one per lane of a
+ * wave64) instead of 128 (two per lane, the default unless the tuner
+ * measured 64 faster for the component).                                   */
+#define BCX_FLAG_TILE64          64
 
 typedef struct {
     double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */
@@ -303,17 +307,18 @@ int  bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src,
 #define BCX_KERNEL_WG        2   /* workgroup per tile, state in HBM (deep BFS)   */
 #define BCX_KERNEL_MULTI     3   /* level launches across the device per tile     */
 
-/* Parity probe of one whole tile of the batched path: srcs[0..ns) (1 <= ns
- * <= 64, distinct local vertices) run as the lanes of one tile on the kernel
- * that bcx_run picks for this component under opts->flags (the flags force
- * a kernel as they do for a run), with that kernel's per-lane probe arrays
- * switched on; every lane is read back.  Outputs are [ns * n], row i for
- * srcs[i] (any pointer may be NULL): levels, path counts (both forms; the
- * tile's counts are FP64 when *sigma_is_f64 = 1, else exact integers) and
+/* Parity probe of one whole tile of the batched path: srcs[0..ns) (distinct
+ * local vertices; 1 <= ns <= 64, or <= 128 on the multi-launch tiles unless
+ * BCX_FLAG_TILE64) run as the sources of one tile on the kernel that bcx_run
+ * picks for this component under opts->flags (the flags force a kernel as
+ * they do for a run), with that kernel's per-source probe arrays switched
+ * on; every source is read back.  Outputs are [ns * n], row i for srcs[i]
+ * (any pointer may be NULL): levels, path counts (both forms; the tile's
+ * counts are FP64 when *sigma_is_f64 = 1, else exact integers) and
  * dependencies.  *kernel_used receives the BCX_KERNEL_* that ran.  Extends
  * bcx_debug_source (one source; the reference's per-source rows are
- * Codes/kernels_active.cu:544-552) to the 64 lanes of a tile, which the
- * reference, one source per block, does not have.                         */
+ * Codes/kernels_active.cu:544-552) to the sources of a tile, which the
+ * reference, one source per block, does not have.                          */
 int  bcx_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns,
                     const bcx_opts *opts, int32_t *level, uint64_t *sigma_u64,
                     double *sigma_f64, double *delta, int32_t *sigma_is_f64,

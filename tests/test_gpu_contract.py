@@ -45,41 +45,57 @@ def contracted(engine, c):
     return st["n_contracted"]
 
 
-def chain_rich_tiles(c, rng):
-    """Tiles of 64 sources: chain interiors only, and interiors mixed with
-    their endpoints and other vertices."""
+def chain_rich_tiles(c, rng, width=64):
+    """Tiles of `width` sources: chain interiors only, and interiors mixed
+    with their endpoints and other vertices."""
     nodes = np.asarray(c.chain_nodes)
     ends = np.unique(np.asarray(c.chain_ends))
-    t1 = rng.choice(nodes, size=min(64, len(nodes)), replace=False)
-    mix = np.concatenate([rng.choice(nodes, size=min(24, len(nodes)), replace=False),
-                          rng.choice(ends, size=min(20, len(ends)), replace=False)])
+    t1 = rng.choice(nodes, size=min(width, len(nodes)), replace=False)
+    mix = np.concatenate([rng.choice(nodes, size=min(3 * width // 8, len(nodes)), replace=False),
+                          rng.choice(ends, size=min(5 * width // 16, len(ends)), replace=False)])
     rest = np.setdiff1d(np.arange(c.n), mix)
-    t2 = np.concatenate([mix, rng.choice(rest, size=64 - len(mix), replace=False)])
+    t2 = np.concatenate([mix, rng.choice(rest, size=width - len(mix), replace=False)])
     return [t1, t2]
 
 
+# tiles of 64 sources (one per lane) and of 128 (two per lane)
+WIDTHS = {64: MULTI | bcx.FLAG_TILE64, 128: MULTI}
+
+
+@pytest.mark.parametrize("width", [64, 128])
 @pytest.mark.parametrize("prec", [bcx.PREC_FP64, bcx.PREC_FP32C])
-def test_theta_every_source(engine, prec):
+def test_theta_every_source(engine, prec, width):
     """All 29 vertices of the theta graph as one tile: every position of every
-    chain is a source once, against the oracle lane by lane."""
+    chain is a source once, against the oracle lane by lane (with 128-source
+    tiles, the second word of each lane holds the reversed order)."""
     n, e = theta_graph()
     (c,) = comps_of(n, e)
     assert c.n_chains == 6
     assert contracted(engine, c) == 23
-    check_tile(engine, c, np.arange(c.n), prec, MULTI, bcx.KERNEL_MULTI, tag="theta")
-    check_tile(engine, c, np.arange(c.n)[::-1], prec, MULTI, bcx.KERNEL_MULTI, tag="theta/rev")
+    flags = WIDTHS[width]
+    check_tile(engine, c, np.arange(c.n), prec, flags, bcx.KERNEL_MULTI, tag="theta")
+    check_tile(engine, c, np.arange(c.n)[::-1], prec, flags, bcx.KERNEL_MULTI, tag="theta/rev")
+    if width == 128:
+        # 64 + 29: lanes 0..28 carry a source in both words
+        rng = np.random.default_rng(1)
+        big = max(comps_of(*bcx.generate(2, 0, 0.003)), key=lambda c: c.n)
+        srcs = np.concatenate([rng.choice(big.n, size=64, replace=False)])
+        check_tile(engine, big, np.concatenate([srcs, rng.choice(np.setdiff1d(np.arange(big.n), srcs), 29,
+                                                                 replace=False)]),
+                   prec, flags, bcx.KERNEL_MULTI, tag="mixed93")
 
 
+@pytest.mark.parametrize("width", [64, 128])
 @pytest.mark.parametrize("cfg,scale", [(2, 0.003), (5, 0.003), (1, 1.0)])
 @pytest.mark.parametrize("prec", [bcx.PREC_FP64, bcx.PREC_FP32C])
-def test_contracted_generated_tiles(engine, cfg, scale, prec):
+def test_contracted_generated_tiles(engine, cfg, scale, prec, width):
     n, uv = bcx.generate(cfg, 0, scale)
     c = max(comps_of(n, uv), key=lambda c: c.n)
-    print("case cfg", cfg, "n", c.n, "chains", c.n_chains, flush=True)
+    print("case cfg", cfg, "n", c.n, "chains", c.n_chains, "width", width, flush=True)
     assert contracted(engine, c) > 0
     rng = np.random.default_rng(cfg)
-    for k, srcs in enumerate(chain_rich_tiles(c, rng)):
-        check_tile(engine, c, srcs, prec, MULTI, bcx.KERNEL_MULTI, tag=f"cfg{cfg}/t{k}")
+    for k, srcs in enumerate(chain_rich_tiles(c, rng, width)):
+        check_tile(engine, c, srcs, prec, WIDTHS[width], bcx.KERNEL_MULTI, tag=f"cfg{cfg}/t{k}/w{width}")
 
 
 def test_contracted_tile_fp64_counts(engine):
@@ -88,7 +104,7 @@ def test_contracted_tile_fp64_counts(engine):
     counts stay bit-exact."""
     n, uv = bcx.generate(2, 0, 0.003)
     c = max(comps_of(n, uv), key=lambda c: c.n)
-    srcs = chain_rich_tiles(c, np.random.default_rng(7))[1]
+    srcs = chain_rich_tiles(c, np.random.default_rng(7), 128)[1]
     lv, su, sf, dl, isf, kern = engine.debug_tile(c, srcs, precision=bcx.PREC_FP64, flags=MULTI,
                                                   sigma_threshold_log2=3)
     assert isf and kern == bcx.KERNEL_MULTI

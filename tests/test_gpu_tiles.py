@@ -24,7 +24,9 @@ pytestmark = pytest.mark.gpu
 SMALL = 0
 WG = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_WG_KERNEL
 MULTI = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_NO_WG_KERNEL
-KERNELS = {"small": (SMALL, bcx.KERNEL_SMALL), "wg": (WG, bcx.KERNEL_WG), "multi": (MULTI, bcx.KERNEL_MULTI)}
+# the multi-launch tiles hold 128 sources (two per lane) unless TILE64
+KERNELS = {"small": (SMALL, bcx.KERNEL_SMALL), "wg": (WG, bcx.KERNEL_WG), "multi": (MULTI, bcx.KERNEL_MULTI),
+           "multi64": (MULTI | bcx.FLAG_TILE64, bcx.KERNEL_MULTI)}
 
 
 def comps_of(n, uv, chains=True):
@@ -48,7 +50,27 @@ def test_golden_tiles(engine, goldens, prec, kernel):
             check_tile(engine, c, srcs, prec, flags, kid, tag=f"{name}/{kernel}")
 
 
-@pytest.mark.parametrize("kernel", ["small", "wg", "multi"])
+@pytest.mark.parametrize("cfg,scale", [(1, 0.1), (2, 0.002), (4, 0.02), (5, 0.002)])
+def test_generated_wide_tiles(engine, cfg, scale):
+    """Whole 128-source tiles of the multi-launch kernels (two sources per
+    lane): every source against the oracle."""
+    flags, kid = KERNELS["multi"]
+    n, uv = bcx.generate(cfg, 0, scale)
+    comps = comps_of(n, uv)
+    c = comps[int(np.argmax([c.n for c in comps]))]
+    g = engine.upload([c])
+    try:
+        order = g.batched_order(0)
+    finally:
+        g.free()
+    T = (c.n + 127) // 128
+    for t in sorted({0, T // 2, T - 1}):
+        srcs = order[t * 128:(t + 1) * 128]
+        check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, tag=f"cfg{cfg}/w{t}")
+    check_tile(engine, c, order[:128], bcx.PREC_FP64, flags, kid, tag=f"cfg{cfg}/w0/fp64")
+
+
+@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64"])
 @pytest.mark.parametrize("cfg,scale", [(1, 0.1), (2, 0.002), (4, 0.02), (5, 0.002)])
 def test_generated_tiles(engine, cfg, scale, kernel):
     """Whole tuned tiles (bcx_batched_order) of the largest component of each
@@ -71,7 +93,7 @@ def test_generated_tiles(engine, cfg, scale, kernel):
         check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, tag=f"cfg{cfg}/t{t}/{kernel}")
 
 
-@pytest.mark.parametrize("kernel", ["small", "wg", "multi"])
+@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64"])
 def test_fp64_counts_grid_tile(engine, kernel):
     """40x40 grid: corner path counts C(78,39) > 2^64, so a tile holding a
     corner runs with FP64 counts on every kernel, bit-exact in pull order."""
@@ -81,6 +103,9 @@ def test_fp64_counts_grid_tile(engine, kernel):
     srcs = [gid.index(v) for v in (0, 39, 1560, 1599)] + [gid.index(v) for v in range(41, 41 + 60)]
     check_tile(engine, c, srcs, bcx.PREC_FP64, flags, kid, want_f64=True, tag=f"grid40/{kernel}")
     check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, want_f64=True, tag=f"grid40/{kernel}/fp32c")
+    if kernel == "multi":
+        wide = srcs + [v for v in range(400, 464)]
+        check_tile(engine, c, wide, bcx.PREC_FP32C, flags, kid, want_f64=True, tag="grid40/multi/128")
 
 
 @pytest.mark.parametrize("kernel", ["wg", "multi"])
