@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--summarize-trace", default="",
                     help="rocprofv3 kernel-trace CSV: print per-kernel calls, summed and union busy time")
     ap.add_argument("--slots", type=int, default=0, help="concurrent tiles of the batched path (0 = library default)")
+    ap.add_argument("--spl", type=int, default=0,
+                    help="sources per lane of the multi-launch tiles (1, 2, 4; 0 = tuned), for A/B measurements")
     ap.add_argument("--flags", type=int, default=0,
                     help="BCX_FLAG_* bits of the batched run, for A/B measurements (16 = no chain contraction)")
     args = ap.parse_args()
@@ -236,7 +238,7 @@ def main():
     idx = [i % args.slices for i in range(K)]
     t_b, tk_b, st_b = timed_steps(g, slices, idx, args.warmup, path=bcx.PATH_BATCHED,
                                   precision=bcx.PREC_FP32C, use_chains=True, batch_width=args.slots,
-                                  flags=args.flags)
+                                  flags=args.flags, sources_per_lane=args.spl)
     src_b = sum(len(slices[i][0]) for i in idx)
     work_b = W * src_b / total_src
     mteps_b = work_b / t_b / 1e6
@@ -306,6 +308,7 @@ def main():
              "components": {"small": st_b[0]["n_comp_small"], "wg": st_b[0]["n_comp_wg"],
                             "multi": st_b[0]["n_comp_multi"]},
              "multi_blocks_per_cu": st_b[0]["sched_blocks_per_cu"], "multi_slots": st_b[0]["sched_slots"],
+             "multi_sources_per_lane": st_b[0]["sched_sources_per_lane"],
              "multi_launch_tiles_per_step": tiles_run / K,
              "list_entries_per_tile": round(sum(st["n_list_entries"] for st in st_b) / tiles_run, 1)
              if tiles_run else None}

@@ -47,7 +47,7 @@ class Opts(C.Structure):
     _fields_ = [("path", C.c_int32), ("precision", C.c_int32), ("use_chains", C.c_int32),
                 ("batch_width", C.c_int32), ("src_subset", _i64p), ("n_src", C.c_int64),
                 ("row_cap_bytes", C.c_int64), ("sigma_f64_threshold_log2", C.c_int32),
-                ("flags", C.c_int32), ("blocks_per_cu", C.c_int32)]
+                ("flags", C.c_int32), ("blocks_per_cu", C.c_int32), ("sources_per_lane", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -58,7 +58,8 @@ class Stats(C.Structure):
                 ("teps_work", C.c_double), ("t_chain_s", C.c_double), ("n_graph_launches", C.c_int64),
                 ("n_contracted", C.c_int64), ("n_list_entries", C.c_int64), ("n_tiles", C.c_int64),
                 ("n_comp_small", C.c_int32), ("n_comp_wg", C.c_int32), ("n_comp_multi", C.c_int32),
-                ("n_tuned", C.c_int32), ("sched_blocks_per_cu", C.c_int32), ("sched_slots", C.c_int32)]
+                ("n_tuned", C.c_int32), ("sched_blocks_per_cu", C.c_int32), ("sched_slots", C.c_int32),
+                ("sched_sources_per_lane", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -315,17 +316,18 @@ FLAG_NO_WG_KERNEL = 4
 FLAG_WG_KERNEL = 8
 FLAG_NO_CONTRACT = 16
 FLAG_NO_TUNE = 32
-FLAG_TILE64 = 64
 KERNEL_SMALL, KERNEL_WG, KERNEL_MULTI = 1, 2, 3
 
 
 def make_opts(path=PATH_PLAIN, precision=PREC_FP64, use_chains=True, subset=None,
-              row_cap_bytes=0, sigma_threshold_log2=0, batch_width=0, flags=0, blocks_per_cu=0):
+              row_cap_bytes=0, sigma_threshold_log2=0, batch_width=0, flags=0, blocks_per_cu=0,
+              sources_per_lane=0):
     o = Opts()
     o.path, o.precision, o.use_chains = path, precision, int(use_chains)
     o.batch_width = batch_width
     o.flags = flags
     o.blocks_per_cu = blocks_per_cu
+    o.sources_per_lane = sources_per_lane
     o.row_cap_bytes = row_cap_bytes
     o.sigma_f64_threshold_log2 = sigma_threshold_log2
     keep = None

@@ -9,9 +9,9 @@
 namespace bcx {
 
 constexpr int BATCH_W = 64;  // lanes of a wave64; a tile holds W * 64 sources
-constexpr int MAX_SPL = 2;   // largest W: sources per lane of the multi-launch tiles
+constexpr int MAX_SPL = 4;   // largest W: sources per lane of the multi-launch tiles
 
-// A tile of the multi-launch kernels holds W * 64 sources (W = spl, 1 or 2):
+// A tile of the multi-launch kernels holds W * 64 sources (W = spl: 1, 2 or 4):
 // lane k owns sources k + 64 h, h < W.  A source mask is W 64-bit words (word
 // h bit k = source k + 64 h), stored as W consecutive words per vertex or list
 // entry; a per-(vertex, source) row holds W consecutive words per lane, row

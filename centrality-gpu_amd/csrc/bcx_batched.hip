@@ -1,5 +1,5 @@
 // Batched multi-source Brandes for gfx950: one tile = one component and up
-// to 64 W sources (W = 1 or 2 per lane of a wave64: lane k owns sources
+// to 64 W sources (W = 1, 2 or 4 per lane of a wave64: lane k owns sources
 // k + 64 h).  State is vertex-major so a single gather serves every source of
 // the tile:
 //   vis[v]        W u64  sources that have reached v
@@ -52,7 +52,7 @@ namespace bcx {
 // measured 2-3.5% above 256 on power-law graphs, 512 5-7% below 256
 constexpr int BB = 128;
 constexpr int NW = BB / 64;    // waves per block
-constexpr int UNR = 8;         // gathers in flight per lane in the sum loops
+constexpr int UNR = 8;         // gathers in flight per lane in the sum loops (4 with W = 4)
 constexpr int SMALL = 16;      // degree up to which one lane expands an entry
 constexpr int STAGE = 256, STAGE_FLUSH = 192;
 constexpr int FEB = 8;         // list entries per wave block in the backward
@@ -127,24 +127,30 @@ __device__ __forceinline__ Msk<W> mshfl(const Msk<W> &a, int src) {
 template <int W>
 __device__ __forceinline__ Msk<W> mld(const uint64_t *p, int64_t i) {
     Msk<W> r;
-    if constexpr (W == 2) {
-        const ulonglong2 y = ((const ulonglong2 *)p)[i];
-        r.m[0] = y.x;
-        r.m[1] = y.y;
-    } else {
+    if constexpr (W == 1) {
         r.m[0] = p[i];
+    } else {
+#pragma unroll
+        for (int q = 0; q < W / 2; ++q) {
+            const ulonglong2 y = ((const ulonglong2 *)p)[i * (W / 2) + q];
+            r.m[2 * q] = y.x;
+            r.m[2 * q + 1] = y.y;
+        }
     }
     return r;
 }
 template <int W>
 __device__ __forceinline__ void mst(uint64_t *p, int64_t i, const Msk<W> &a) {
-    if constexpr (W == 2) {
-        ulonglong2 y;
-        y.x = a.m[0];
-        y.y = a.m[1];
-        ((ulonglong2 *)p)[i] = y;
-    } else {
+    if constexpr (W == 1) {
         p[i] = a.m[0];
+    } else {
+#pragma unroll
+        for (int q = 0; q < W / 2; ++q) {
+            ulonglong2 y;
+            y.x = a.m[2 * q];
+            y.y = a.m[2 * q + 1];
+            ((ulonglong2 *)p)[i * (W / 2) + q] = y;
+        }
     }
 }
 // fire-and-forget OR of the non-zero words (results unused: no return)
@@ -182,32 +188,49 @@ template <int SM>
 __device__ __forceinline__ uint64_t sg_max() {
     return SM == SIG_U32 ? 0xffffffffull : ~0ull;
 }
-// the W sigma words of lane cell c = v * 64 + k, one load
+// the W sigma words of lane cell c = v * 64 + k (one 8- or 16-byte load per
+// pair or quad of words)
 template <int SM, int W>
 __device__ __forceinline__ void sg_row(const BatchDev &b, int64_t c, uint64_t (&x)[W]) {
-    if constexpr (W == 2) {
-        if constexpr (SM == SIG_U32) {
-            const uint2 y = ((const uint2 *)b.sig)[c];
-            x[0] = y.x;
-            x[1] = y.y;
-        } else {
-            const ulonglong2 y = ((const ulonglong2 *)b.sig)[c];
-            x[0] = y.x;
-            x[1] = y.y;
-        }
-    } else {
+    if constexpr (W == 1) {
         x[0] = sg<SM>(b, c);
+    } else if constexpr (SM == SIG_U32 && W == 4) {
+        const uint4 y = ((const uint4 *)b.sig)[c];
+        x[0] = y.x;
+        x[1] = y.y;
+        x[2] = y.z;
+        x[3] = y.w;
+    } else if constexpr (SM == SIG_U32) {
+        const uint2 y = ((const uint2 *)b.sig)[c];
+        x[0] = y.x;
+        x[1] = y.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < W / 2; ++q) {
+            const ulonglong2 y = ((const ulonglong2 *)b.sig)[c * (W / 2) + q];
+            x[2 * q] = y.x;
+            x[2 * q + 1] = y.y;
+        }
     }
 }
 template <typename TT, int W>
 __device__ __forceinline__ void tt_row(const BatchDev &b, int64_t c, TT (&x)[W]) {
-    if constexpr (W == 2) {
-        using V2 = typename std::conditional<sizeof(TT) == 4, float2, double2>::type;
-        const V2 y = ((const V2 *)b.tt)[c];
+    if constexpr (W == 1) {
+        x[0] = ((const TT *)b.tt)[c];
+    } else if constexpr (sizeof(TT) == 4 && W == 4) {
+        const float4 y = ((const float4 *)b.tt)[c];
         x[0] = y.x;
         x[1] = y.y;
+        x[2] = y.z;
+        x[3] = y.w;
     } else {
-        x[0] = ((const TT *)b.tt)[c];
+        using V2 = typename std::conditional<sizeof(TT) == 4, float2, double2>::type;
+#pragma unroll
+        for (int q = 0; q < W / 2; ++q) {
+            const V2 y = ((const V2 *)b.tt)[c * (W / 2) + q];
+            x[2 * q] = y.x;
+            x[2 * q + 1] = y.y;
+        }
     }
 }
 
@@ -405,6 +428,7 @@ template <int SM, int W>
 __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, const Msk<W> &N, int a0, int a1,
                                           uint64_t (&acc)[W], double (&facc)[W], bool &ovf, const int (&sc)[W],
                                           const Msk<W> &cm) {
+    constexpr int UW = W == 4 ? UNR / 2 : UNR;  // gathers in flight (W words each)
     const int lane = threadIdx.x & 63;
     for (int c = a0; c < a1; c += 64) {
         const int j = c + lane;
@@ -416,10 +440,10 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
         }
         uint64_t bal = __ballot(many<W>(f));
         while (bal) {
-            uint64_t x[UNR][W];
-            int use[UNR];
+            uint64_t x[UW][W];
+            int use[UW];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
+            for (int u = 0; u < UW; ++u) {
                 use[u] = 0;
 #pragma unroll
                 for (int h = 0; h < W; ++h) x[u][h] = 0;
@@ -444,7 +468,7 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
                 }
             }
 #pragma unroll
-            for (int u = 0; u < UNR; ++u)
+            for (int u = 0; u < UW; ++u)
 #pragma unroll
                 for (int h = 0; h < W; ++h) {
                     if (!((use[u] >> h) & 1)) continue;
@@ -699,6 +723,7 @@ template <typename TT, int W>
 __device__ __forceinline__ void rho_sum(const BatchDev &b, const uint64_t *succ, const Msk<W> &M, int a0, int a1,
                                         double (&out)[W]) {
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanF, SumD>::type;
+    constexpr int UW = W == 4 ? UNR / 2 : UNR;
     const int lane = threadIdx.x & 63;
     Acc acc[W];
     for (int c = a0; c < a1; c += 64) {
@@ -711,10 +736,10 @@ __device__ __forceinline__ void rho_sum(const BatchDev &b, const uint64_t *succ,
         }
         uint64_t bal = __ballot(many<W>(x));
         while (bal) {
-            TT t[UNR][W];
-            int use[UNR];
+            TT t[UW][W];
+            int use[UW];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
+            for (int u = 0; u < UW; ++u) {
                 use[u] = 0;
 #pragma unroll
                 for (int h = 0; h < W; ++h) t[u][h] = 0;
@@ -727,7 +752,7 @@ __device__ __forceinline__ void rho_sum(const BatchDev &b, const uint64_t *succ,
                 }
             }
 #pragma unroll
-            for (int u = 0; u < UNR; ++u)
+            for (int u = 0; u < UW; ++u)
 #pragma unroll
                 for (int h = 0; h < W; ++h)
                     if ((use[u] >> h) & 1) acc[h].add(t[u][h]);
@@ -772,6 +797,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     __shared__ int t_k[NW][WIN];
     // entries per wave block: FEB / W, so a block holds FEB * 64 (entry, source) pairs
     constexpr int EB = FEB / W;
+    constexpr int UW = W == 4 ? UNR / 2 : UNR;
     __shared__ double r_rho[NW][EB][64 * W];  // [entry][h * 64 + lane]
     // the host launches the backward speculatively behind the forward: a
     // level beyond the depth, an unfinished forward (depth -1), a sigma
@@ -864,11 +890,11 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                 nrel += __popcll(bal);
             }
             __builtin_amdgcn_wave_barrier();
-            for (int r = 0; r < nrel; r += UNR) {
-                TT t[UNR][W];
-                int kk[UNR], use[UNR];
+            for (int r = 0; r < nrel; r += UW) {
+                TT t[UW][W];
+                int kk[UW], use[UW];
 #pragma unroll
-                for (int u = 0; u < UNR; ++u) {
+                for (int u = 0; u < UW; ++u) {
                     const int rr = r + u;
                     use[u] = 0;
                     kk[u] = -1;
@@ -881,7 +907,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < UNR; ++u) {
+                for (int u = 0; u < UW; ++u) {
                     if (kk[u] < 0) continue;
                     if (kk[u] != kcur) {
                         if (kcur >= 0)
@@ -1213,7 +1239,8 @@ static void init_w(const BatchDev &b, int ns, int sm, int g, hipStream_t st) {
 
 hipError_t launch_b_init(const BatchDev &b, int ns, int sm, int cap, hipStream_t st) {
     const int g = (int)std::min<int64_t>(((int64_t)b.n + BB - 1) / BB, cap);
-    if (b.spl == 2) init_w<2>(b, ns, sm, g, st);
+    if (b.spl == 4) init_w<4>(b, ns, sm, g, st);
+    else if (b.spl == 2) init_w<2>(b, ns, sm, g, st);
     else init_w<1>(b, ns, sm, g, st);
     return hipGetLastError();
 }
@@ -1227,7 +1254,8 @@ static void forward_w(const BatchDev &b, int sm, int l, int grid, hipStream_t st
 }
 
 hipError_t launch_b_forward(const BatchDev &b, int sm, int l, int grid, hipStream_t st) {
-    if (b.spl == 2) forward_w<2>(b, sm, l, grid, st);
+    if (b.spl == 4) forward_w<4>(b, sm, l, grid, st);
+    else if (b.spl == 2) forward_w<2>(b, sm, l, grid, st);
     else forward_w<1>(b, sm, l, grid, st);
     return hipGetLastError();
 }
@@ -1245,13 +1273,15 @@ static void back_w(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStr
 }
 
 hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st) {
-    if (b.spl == 2) back_w<2>(b, sm, fp32, l, grid, st);
+    if (b.spl == 4) back_w<4>(b, sm, fp32, l, grid, st);
+    else if (b.spl == 2) back_w<2>(b, sm, fp32, l, grid, st);
     else back_w<1>(b, sm, fp32, l, grid, st);
     return hipGetLastError();
 }
 
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st) {
-    if (b.spl == 2) hipLaunchKernelGGL(k_b_finish<2>, dim3(grid), dim3(BB), 0, st, b);
+    if (b.spl == 4) hipLaunchKernelGGL(k_b_finish<4>, dim3(grid), dim3(BB), 0, st, b);
+    else if (b.spl == 2) hipLaunchKernelGGL(k_b_finish<2>, dim3(grid), dim3(BB), 0, st, b);
     else hipLaunchKernelGGL(k_b_finish<1>, dim3(grid), dim3(BB), 0, st, b);
     return hipGetLastError();
 }
@@ -1269,7 +1299,8 @@ static void chain_check_w(const BatchDev &b, int sm, int g, hipStream_t st) {
 
 hipError_t launch_b_chain_check(const BatchDev &b, int sm, int grid, hipStream_t st) {
     (void)grid;
-    if (b.spl == 2) chain_check_w<2>(b, sm, chain_grid(b), st);
+    if (b.spl == 4) chain_check_w<4>(b, sm, chain_grid(b), st);
+    else if (b.spl == 2) chain_check_w<2>(b, sm, chain_grid(b), st);
     else chain_check_w<1>(b, sm, chain_grid(b), st);
     return hipGetLastError();
 }
@@ -1285,7 +1316,8 @@ static void chains_w(const BatchDev &b, int sm, bool fp32, int g, hipStream_t st
 
 hipError_t launch_b_chains(const BatchDev &b, int sm, bool fp32, int grid, hipStream_t st) {
     (void)grid;
-    if (b.spl == 2) chains_w<2>(b, sm, fp32, chain_grid(b), st);
+    if (b.spl == 4) chains_w<4>(b, sm, fp32, chain_grid(b), st);
+    else if (b.spl == 2) chains_w<2>(b, sm, fp32, chain_grid(b), st);
     else chains_w<1>(b, sm, fp32, chain_grid(b), st);
     return hipGetLastError();
 }
@@ -1297,7 +1329,8 @@ hipError_t launch_b_fold(double *dst, double *src, const int32_t *perm, int n, i
 
 hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st) {
     if (end <= 0) return hipSuccess;
-    if (b.spl == 2) hipLaunchKernelGGL(k_b_clear_all<2>, dim3(grid), dim3(BB), 0, st, b, end);
+    if (b.spl == 4) hipLaunchKernelGGL(k_b_clear_all<4>, dim3(grid), dim3(BB), 0, st, b, end);
+    else if (b.spl == 2) hipLaunchKernelGGL(k_b_clear_all<2>, dim3(grid), dim3(BB), 0, st, b, end);
     else hipLaunchKernelGGL(k_b_clear_all<1>, dim3(grid), dim3(BB), 0, st, b, end);
     return hipGetLastError();
 }

@@ -1169,15 +1169,19 @@ struct MultiSched {
     int spl;  // sources per lane
 };
 
-// sources per lane unless tuned or fixed: two (128-source tiles) measured
-// above one on the Chung-Lu and R-MAT cores (fewer entries and arc walks per
-// source: a tile reaches a vertex at few more distinct levels)
-constexpr int DEFAULT_SPL = 2;
+// sources per lane unless tuned or fixed: four (256-source tiles) measured
+// above two and two above one on the Chung-Lu and R-MAT cores (fewer entries
+// and arc walks per source: a tile reaches a vertex at few more distinct
+// levels): 165.0k / 147.6k / 126.0k MTEPS on the 1M-vertex mixed graph,
+// 276.9k / 244.1k / 214.2k on R-MAT scale 20
+constexpr int DEFAULT_SPL = 4;
+
+static bool valid_spl(int w) { return w == 1 || w == 2 || w == 4; }
 
 static MultiSched multi_sched(const DComp &dc, const bcx_opts *o) {
     MultiSched m;
     m.cut = want_cut(o) && dc.sched_cut != 0;
-    m.spl = (o->flags & BCX_FLAG_TILE64) ? 1 : (dc.sched_spl > 0 ? dc.sched_spl : DEFAULT_SPL);
+    m.spl = valid_spl(o->sources_per_lane) ? o->sources_per_lane : (dc.sched_spl > 0 ? dc.sched_spl : DEFAULT_SPL);
     m.bpc = o->blocks_per_cu > 0 ? std::min(o->blocks_per_cu, 64) : (dc.sched_bpc > 0 ? dc.sched_bpc : DEFAULT_BPC);
     m.slots = o->batch_width > 0 ? std::max(1, std::min(MAX_SLOTS, o->batch_width))
                                  : (dc.sched_slots > 0 ? dc.sched_slots : MAX_SLOTS);
@@ -1213,10 +1217,11 @@ int batched_component(bcx_ctx *ctx, DComp &dc, const std::vector<int32_t> &srcs,
     return multi_run(ctx, dc, srcs, o, multi_sched(dc, o), bc, st);
 }
 
-void batched_sched_report(const DComp &dc, const bcx_opts *o, int32_t *bpc, int32_t *slots) {
+void batched_sched_report(const DComp &dc, const bcx_opts *o, int32_t *bpc, int32_t *slots, int32_t *spl) {
     const MultiSched m = multi_sched(dc, o);
     *bpc = m.bpc;
     *slots = m.slots;
+    *spl = m.spl;
 }
 
 // ------------------------------------------------------------------------
@@ -1596,7 +1601,9 @@ int batched_schedule(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, bcx_stats *st) 
             for (int bpc : {4, 16}) cands.push_back({base.cut, bpc, base.slots, base.spl});
         if (o->batch_width <= 0) cands.push_back({base.cut, base.bpc, 4, base.spl});
         if (base.cut) cands.push_back({false, base.bpc, base.slots, base.spl});
-        if (!(o->flags & BCX_FLAG_TILE64)) cands.push_back({base.cut, base.bpc, base.slots, 3 - base.spl});
+        if (!valid_spl(o->sources_per_lane))
+            for (int w : {1, 2, 4})
+                if (w != base.spl) cands.push_back({base.cut, base.bpc, base.slots, w});
         double r_best = r_multi;
         for (const MultiSched &m : cands) {
             double r = 0.0;
@@ -1617,7 +1624,7 @@ int batched_schedule(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, bcx_stats *st) 
         dc.sched_multi = true;
         dc.sched_cut = (base.cut && !best.cut) ? 0 : 1;
         dc.sched_bpc = (int16_t)(o->blocks_per_cu > 0 ? 0 : best.bpc);
-        dc.sched_spl = (int8_t)((o->flags & BCX_FLAG_TILE64) ? 0 : best.spl);
+        dc.sched_spl = (int8_t)(valid_spl(o->sources_per_lane) ? 0 : best.spl);
         dc.sched_slots = (int16_t)(o->batch_width > 0 ? 0 : best.slots);
     }
     if (st) st->n_tuned++;

@@ -80,7 +80,7 @@ struct DComp {
     int8_t sched_cut = 1;     // multi-launch: 1 contracted chains, 0 the relabelled copy
     int16_t sched_bpc = 0;    // multi-launch: level-kernel blocks per CU (0: default)
     int16_t sched_slots = 0;  // multi-launch: tiles in flight (0: default)
-    int8_t sched_spl = 0;     // multi-launch: sources per lane, 1 or 2 (0: default)
+    int8_t sched_spl = 0;     // multi-launch: sources per lane, 1, 2 or 4 (0: default)
 };
 
 // One endpoint-row batch: BFS work (endpoints, with slot and mode) then the
@@ -183,8 +183,9 @@ int batched_kind(bcx::DComp &dc, int flags);
 // TUNE_N vertices (unless the options fix it), else batched_kind's rule.
 // Returns a BCX_KERNEL_* or a negative BCX_E_* status.
 int batched_schedule(bcx_ctx *ctx, bcx::DComp &dc, const bcx_opts *o, bcx_stats *st);
-// Blocks per CU and tiles in flight a multi-launch run of dc uses.
-void batched_sched_report(const bcx::DComp &dc, const bcx_opts *o, int32_t *bpc, int32_t *slots);
+// Blocks per CU, tiles in flight and sources per lane a multi-launch run of
+// dc uses.
+void batched_sched_report(const bcx::DComp &dc, const bcx_opts *o, int32_t *bpc, int32_t *slots, int32_t *spl);
 int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *srcs, int32_t ns, const bcx_opts *o,
                        int32_t *level, uint64_t *sigma_u64, double *sigma_f64, double *delta, int32_t *sigma_is_f64,
                        int32_t *kernel_used);

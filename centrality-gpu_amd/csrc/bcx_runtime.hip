@@ -664,7 +664,7 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
             s->n_comp_multi++;
             if (dc.n > largest_multi) {
                 largest_multi = dc.n;
-                batched_sched_report(dc, &o, &s->sched_blocks_per_cu, &s->sched_slots);
+                batched_sched_report(dc, &o, &s->sched_blocks_per_cu, &s->sched_slots, &s->sched_sources_per_lane);
             }
             multi.push_back({&dc, std::move(tiles)});
         }

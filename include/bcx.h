@@ -157,6 +157,9 @@ typedef struct {
     int32_t        flags;         /* BCX_FLAG_* bits                          */
     int32_t        blocks_per_cu; /* batched multi-launch tiles: level-kernel
                                      blocks per compute unit (0 = tuned)     */
+    int32_t        sources_per_lane; /* batched multi-launch tiles: 1, 2 or 4
+                                     sources per lane of a wave64, tiles of
+                                     64, 128 or 256 sources (0 = tuned)      */
 } bcx_opts;
 
 /* Batched path: run components of up to 2048 vertices through the
@@ -188,15 +191,13 @@ typedef struct {
  * reference, which switches schedule from sampled BFS depths,
  * Codes/kernels_free.cu:447-470): the workgroup or multi-launch kernel for
  * components of up to 131072 vertices, and for multi-launch components the
- * level kernels' blocks per CU, the tiles in flight and chain contraction
- * on or off.  NO_TUNE keeps the defaults (the level-width rule, 8 blocks
- * per CU, 8 tiles, contraction on).  Options that fix a choice (the kernel
- * flags, batch_width, blocks_per_cu, NO_CONTRACT) are kept and not tuned.  */
+ * sources per lane, the level kernels' blocks per CU, the tiles in flight
+ * and chain contraction on or off.  NO_TUNE keeps the defaults (the
+ * level-width rule, 4 sources per lane, 8 blocks per CU, 8 tiles,
+ * contraction on).  Options that fix a choice (the kernel flags,
+ * sources_per_lane, batch_width, blocks_per_cu, NO_CONTRACT) are kept and
+ * not tuned.                                                               */
 #define BCX_FLAG_NO_TUNE         32
-/* Batched path, multi-launch tiles: 64 sources per tile (one per lane of a
- * wave64) instead of 128 (two per lane, the default unless the tuner
- * measured 64 faster for the component).                                   */
-#define BCX_FLAG_TILE64          64
 
 typedef struct {
     double  t_kernel_s;        /* HIP-event time of all GPU work in bcx_run   */
@@ -224,8 +225,9 @@ typedef struct {
     int32_t n_comp_multi;      /*   multi-launch tiles                        */
     int32_t n_tuned;           /* batched: components whose schedule was measured
                                   (the schedule of each is measured once)     */
-    int32_t sched_blocks_per_cu; /* batched: blocks per CU and tiles in flight */
-    int32_t sched_slots;       /*   of the largest multi-launch component     */
+    int32_t sched_blocks_per_cu; /* batched: blocks per CU, tiles in flight and */
+    int32_t sched_slots;       /*   sources per lane of the largest           */
+    int32_t sched_sources_per_lane; /* multi-launch component                 */
 } bcx_stats;
 
 /* Context: device, streams, scratch.  mem_cap (bytes, 0 = 90% of free
@@ -308,8 +310,8 @@ int  bcx_debug_source(bcx_ctx *ctx, const bcx_component *comp, int32_t src,
 #define BCX_KERNEL_MULTI     3   /* level launches across the device per tile     */
 
 /* Parity probe of one whole tile of the batched path: srcs[0..ns) (distinct
- * local vertices; 1 <= ns <= 64, or <= 128 on the multi-launch tiles unless
- * BCX_FLAG_TILE64) run as the sources of one tile on the kernel that bcx_run
+ * local vertices; 1 <= ns <= 64, or up to 64 W on the multi-launch tiles of
+ * W sources per lane) run as the sources of one tile on the kernel that bcx_run
  * picks for this component under opts->flags (the flags force a kernel as
  * they do for a run), with that kernel's per-source probe arrays switched
  * on; every source is read back.  Outputs are [ns * n], row i for srcs[i]

@@ -58,34 +58,34 @@ def chain_rich_tiles(c, rng, width=64):
     return [t1, t2]
 
 
-# tiles of 64 sources (one per lane) and of 128 (two per lane)
-WIDTHS = {64: MULTI | bcx.FLAG_TILE64, 128: MULTI}
+# tiles of 64, 128 and 256 sources (one, two and four per lane)
+WIDTHS = {64: 1, 128: 2, 256: 4}
 
 
-@pytest.mark.parametrize("width", [64, 128])
+@pytest.mark.parametrize("width", [64, 128, 256])
 @pytest.mark.parametrize("prec", [bcx.PREC_FP64, bcx.PREC_FP32C])
 def test_theta_every_source(engine, prec, width):
     """All 29 vertices of the theta graph as one tile: every position of every
-    chain is a source once, against the oracle lane by lane (with 128-source
-    tiles, the second word of each lane holds the reversed order)."""
+    chain is a source once, against the oracle source by source; with wider
+    tiles, a tile of another graph that fills the lanes' later words too."""
     n, e = theta_graph()
     (c,) = comps_of(n, e)
     assert c.n_chains == 6
     assert contracted(engine, c) == 23
-    flags = WIDTHS[width]
-    check_tile(engine, c, np.arange(c.n), prec, flags, bcx.KERNEL_MULTI, tag="theta")
-    check_tile(engine, c, np.arange(c.n)[::-1], prec, flags, bcx.KERNEL_MULTI, tag="theta/rev")
-    if width == 128:
+    spl = WIDTHS[width]
+    check_tile(engine, c, np.arange(c.n), prec, MULTI, bcx.KERNEL_MULTI, tag="theta", spl=spl)
+    check_tile(engine, c, np.arange(c.n)[::-1], prec, MULTI, bcx.KERNEL_MULTI, tag="theta/rev", spl=spl)
+    if width > 64:
         # 64 + 29: lanes 0..28 carry a source in both words
         rng = np.random.default_rng(1)
         big = max(comps_of(*bcx.generate(2, 0, 0.003)), key=lambda c: c.n)
         srcs = np.concatenate([rng.choice(big.n, size=64, replace=False)])
-        check_tile(engine, big, np.concatenate([srcs, rng.choice(np.setdiff1d(np.arange(big.n), srcs), 29,
-                                                                 replace=False)]),
-                   prec, flags, bcx.KERNEL_MULTI, tag="mixed93")
+        extra = rng.choice(np.setdiff1d(np.arange(big.n), srcs), width - 35 - 64, replace=False)
+        check_tile(engine, big, np.concatenate([srcs, extra]), prec, MULTI, bcx.KERNEL_MULTI,
+                   tag=f"mixed{width - 35}", spl=spl)
 
 
-@pytest.mark.parametrize("width", [64, 128])
+@pytest.mark.parametrize("width", [64, 128, 256])
 @pytest.mark.parametrize("cfg,scale", [(2, 0.003), (5, 0.003), (1, 1.0)])
 @pytest.mark.parametrize("prec", [bcx.PREC_FP64, bcx.PREC_FP32C])
 def test_contracted_generated_tiles(engine, cfg, scale, prec, width):
@@ -95,7 +95,7 @@ def test_contracted_generated_tiles(engine, cfg, scale, prec, width):
     assert contracted(engine, c) > 0
     rng = np.random.default_rng(cfg)
     for k, srcs in enumerate(chain_rich_tiles(c, rng, width)):
-        check_tile(engine, c, srcs, prec, WIDTHS[width], bcx.KERNEL_MULTI, tag=f"cfg{cfg}/t{k}/w{width}")
+        check_tile(engine, c, srcs, prec, MULTI, bcx.KERNEL_MULTI, tag=f"cfg{cfg}/t{k}/w{width}", spl=WIDTHS[width])
 
 
 def test_contracted_tile_fp64_counts(engine):

@@ -24,9 +24,10 @@ pytestmark = pytest.mark.gpu
 SMALL = 0
 WG = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_WG_KERNEL
 MULTI = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_NO_WG_KERNEL
-# the multi-launch tiles hold 128 sources (two per lane) unless TILE64
 KERNELS = {"small": (SMALL, bcx.KERNEL_SMALL), "wg": (WG, bcx.KERNEL_WG), "multi": (MULTI, bcx.KERNEL_MULTI),
-           "multi64": (MULTI | bcx.FLAG_TILE64, bcx.KERNEL_MULTI)}
+           "multi64": (MULTI, bcx.KERNEL_MULTI), "multi128": (MULTI, bcx.KERNEL_MULTI)}
+# sources per lane of the multi-launch tiles ("multi": the default, 4)
+SPL = {"multi64": 1, "multi128": 2}
 
 
 def comps_of(n, uv, chains=True):
@@ -47,14 +48,16 @@ def test_golden_tiles(engine, goldens, prec, kernel):
     for name, g in goldens.items():
         for c in comps_of(g["n"], g["edges"], chains=False):
             srcs = np.arange(min(64, c.n))[::-1]  # lanes in reverse id order
-            check_tile(engine, c, srcs, prec, flags, kid, tag=f"{name}/{kernel}")
+            check_tile(engine, c, srcs, prec, flags, kid, tag=f"{name}/{kernel}", spl=SPL.get(kernel, 0))
 
 
+@pytest.mark.parametrize("spl", [2, 4])
 @pytest.mark.parametrize("cfg,scale", [(1, 0.1), (2, 0.002), (4, 0.02), (5, 0.002)])
-def test_generated_wide_tiles(engine, cfg, scale):
-    """Whole 128-source tiles of the multi-launch kernels (two sources per
-    lane): every source against the oracle."""
+def test_generated_wide_tiles(engine, cfg, scale, spl):
+    """Whole tiles of 128 or 256 sources on the multi-launch kernels (two or
+    four sources per lane): every source against the oracle."""
     flags, kid = KERNELS["multi"]
+    tw = 64 * spl
     n, uv = bcx.generate(cfg, 0, scale)
     comps = comps_of(n, uv)
     c = comps[int(np.argmax([c.n for c in comps]))]
@@ -63,14 +66,14 @@ def test_generated_wide_tiles(engine, cfg, scale):
         order = g.batched_order(0)
     finally:
         g.free()
-    T = (c.n + 127) // 128
+    T = (c.n + tw - 1) // tw
     for t in sorted({0, T // 2, T - 1}):
-        srcs = order[t * 128:(t + 1) * 128]
-        check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, tag=f"cfg{cfg}/w{t}")
-    check_tile(engine, c, order[:128], bcx.PREC_FP64, flags, kid, tag=f"cfg{cfg}/w0/fp64")
+        srcs = order[t * tw:(t + 1) * tw]
+        check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, tag=f"cfg{cfg}/w{t}", spl=spl)
+    check_tile(engine, c, order[:tw], bcx.PREC_FP64, flags, kid, tag=f"cfg{cfg}/w0/fp64", spl=spl)
 
 
-@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64"])
+@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64", "multi128"])
 @pytest.mark.parametrize("cfg,scale", [(1, 0.1), (2, 0.002), (4, 0.02), (5, 0.002)])
 def test_generated_tiles(engine, cfg, scale, kernel):
     """Whole tuned tiles (bcx_batched_order) of the largest component of each
@@ -90,10 +93,10 @@ def test_generated_tiles(engine, cfg, scale, kernel):
     T = (c.n + 63) // 64
     for t in sorted({0, T // 2, T - 1}):
         srcs = order[t * 64:(t + 1) * 64]
-        check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, tag=f"cfg{cfg}/t{t}/{kernel}")
+        check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, tag=f"cfg{cfg}/t{t}/{kernel}", spl=SPL.get(kernel, 0))
 
 
-@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64"])
+@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64", "multi128"])
 def test_fp64_counts_grid_tile(engine, kernel):
     """40x40 grid: corner path counts C(78,39) > 2^64, so a tile holding a
     corner runs with FP64 counts on every kernel, bit-exact in pull order."""
@@ -101,11 +104,15 @@ def test_fp64_counts_grid_tile(engine, kernel):
     c = comps_of(1600, grid_edges(40, 40))[0]
     gid = list(c.gid)
     srcs = [gid.index(v) for v in (0, 39, 1560, 1599)] + [gid.index(v) for v in range(41, 41 + 60)]
-    check_tile(engine, c, srcs, bcx.PREC_FP64, flags, kid, want_f64=True, tag=f"grid40/{kernel}")
-    check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, want_f64=True, tag=f"grid40/{kernel}/fp32c")
-    if kernel == "multi":
+    spl = SPL.get(kernel, 0)
+    check_tile(engine, c, srcs, bcx.PREC_FP64, flags, kid, want_f64=True, tag=f"grid40/{kernel}", spl=spl)
+    check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, want_f64=True, tag=f"grid40/{kernel}/fp32c", spl=spl)
+    if kernel == "multi128":
         wide = srcs + [v for v in range(400, 464)]
-        check_tile(engine, c, wide, bcx.PREC_FP32C, flags, kid, want_f64=True, tag="grid40/multi/128")
+        check_tile(engine, c, wide, bcx.PREC_FP32C, flags, kid, want_f64=True, tag="grid40/multi/128", spl=2)
+    if kernel == "multi":
+        wide = srcs + [v for v in range(400, 592)]
+        check_tile(engine, c, wide, bcx.PREC_FP32C, flags, kid, want_f64=True, tag="grid40/multi/256")
 
 
 @pytest.mark.parametrize("kernel", ["wg", "multi"])

@@ -6,11 +6,12 @@ import bcx
 import oracle
 
 
-def check_tile(engine, c, srcs, prec, flags=0, want_kernel=None, want_f64=None, tag=""):
-    """Runs the tile and compares every lane with the oracle; returns the max
+def check_tile(engine, c, srcs, prec, flags=0, want_kernel=None, want_f64=None, tag="", spl=0):
+    """Runs the tile (spl: sources per lane of the multi-launch tiles, 0 =
+    default) and compares every source with the oracle; returns the max
     relative dependency error."""
     srcs = np.asarray(srcs, np.int32)
-    lv, su, sf, dl, isf, kern = engine.debug_tile(c, srcs, precision=prec, flags=flags)
+    lv, su, sf, dl, isf, kern = engine.debug_tile(c, srcs, precision=prec, flags=flags, sources_per_lane=spl)
     if want_kernel is not None:
         assert kern == want_kernel, (tag, kern)
     if want_f64 is not None:
