@@ -1142,12 +1142,16 @@ __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
             wT[h] = F64 ? d2u(sL[h] + sR[h]) : wL[h] + wR[h];
         }
         double p0 = 0.0;  // P(i - 1)
-        // the interiors' BC words, 64 at a time, one per lane (coalesced)
-        double accbc = 0.0;
+        // the interiors' BC words and prefix sums, 64 at a time, one per lane
+        // (coalesced loads; the loop then reads P(i) from lane i - 1)
+        double accbc = 0.0, pv = 0.0;
         for (int i = 1; i < K; ++i) {
             const int64_t x = (int64_t)b.n + off + i - 1;
-            if (((i - 1) & 63) == 0 && i - 1 + lane < K - 1) accbc = b.bc[x + lane];
-            const double p1 = pre[i - 1];  // P(i)
+            if (((i - 1) & 63) == 0) {
+                accbc = i - 1 + lane < K - 1 ? b.bc[x + lane] : 0.0;
+                pv = i - 1 + lane < K - 1 ? pre[i - 1 + lane] : 0.0;
+            }
+            const double p1 = __shfl(pv, (i - 1) & 63, 64);  // P(i)
             double contrib = 0.0;
 #pragma unroll
             for (int h = 0; h < W; ++h) {
