@@ -54,16 +54,30 @@ constexpr int BB = 128;
 constexpr int NW = BB / 64;    // waves per block
 constexpr int UNR = 8;         // gathers in flight per lane in the sum loops (fewer for W = 4)
 constexpr int SMALL = 16;      // degree up to which one lane expands an entry
-constexpr int STAGE = 256, STAGE_FLUSH = 192;
+// The level kernels wait on dependent gathers, so waves per SIMD (occupancy,
+// set by VGPRs and LDS per block) decide their speed: the staging and window
+// sizes keep both kernels' LDS at <= 13.8 KB per block (10+ blocks per CU),
+// so at <= 96 VGPRs they run 5 waves per SIMD instead of 4 (198.6k against
+// 188.9k MTEPS on the 1M-vertex mixed graph with the pull alone at 5 waves,
+// 176.1k with neither)
+constexpr int STAGE = 192, STAGE_FLUSH = 128;  // pull: list entries staged per wave, flush point
 constexpr int FEB = 8;         // list entries per wave block in the backward
-constexpr int WIN = 128;       // arcs per window of a block
+constexpr int WIN = 64;        // arcs per compaction window of a backward block
 
 __device__ __forceinline__ int gwave() { return (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6); }
 __device__ __forceinline__ int nwaves() { return (int)((gridDim.x * (unsigned)blockDim.x) >> 6); }
 
+// x on lane src, for a wave-uniform src (every use here: a ballot's lowest
+// set bit, or a constant): v_readlane into a scalar register, where __shfl
+// would take a round trip through the LDS crossbar (ds_bpermute) into
+// vector registers
+__device__ __forceinline__ int bcast(int x, int src) { return __builtin_amdgcn_readlane(x, src); }
 __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
-    return (uint64_t)__shfl((long long)x, src, 64);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
 }
+__device__ __forceinline__ double bcastd(double x, int src) { return u2d(shfl64(d2u(x), src)); }
 
 __device__ __forceinline__ uint64_t *mbuf(const BatchDev &b, int l) {
     const int r = l % 3;
@@ -188,12 +202,15 @@ template <int SM>
 __device__ __forceinline__ uint64_t sg_max() {
     return SM == SIG_U32 ? 0xffffffffull : ~0ull;
 }
+// register word of a sigma value in mode SM (32 bits for u32 tiles)
+template <int SM>
+using SgW = typename std::conditional<SM == SIG_U32, uint32_t, uint64_t>::type;
 // the W sigma words of lane cell c = v * 64 + k (one 8- or 16-byte load per
 // pair or quad of words)
 template <int SM, int W>
-__device__ __forceinline__ void sg_row(const BatchDev &b, int64_t c, uint64_t (&x)[W]) {
+__device__ __forceinline__ void sg_row(const BatchDev &b, int64_t c, SgW<SM> (&x)[W]) {
     if constexpr (W == 1) {
-        x[0] = sg<SM>(b, c);
+        x[0] = (SgW<SM>)sg<SM>(b, c);
     } else if constexpr (SM == SIG_U32 && W == 4) {
         const uint4 y = ((const uint4 *)b.sig)[c];
         x[0] = y.x;
@@ -399,7 +416,7 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
             const int src = __ffsll((long long)bal) - 1;
             bal &= bal - 1;
             const Msk<W> FF = mshfl<W>(F, src);
-            const int a0 = __shfl(j0, src, 64), a1 = __shfl(j1, src, 64);
+            const int a0 = bcast(j0, src), a1 = bcast(j1, src);
             for (int j = a0 + lane; j < a1; j += 64) {
                 const int w = b.col[j];
                 if (w >= 0) push<W>(b, dst, w, FF);
@@ -428,9 +445,9 @@ template <int SM, int W>
 __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, const Msk<W> &N, int a0, int a1,
                                           uint64_t (&acc)[W], double (&facc)[W], bool &ovf, const int (&sc)[W],
                                           const Msk<W> &cm) {
-    // gathers in flight (W words each): 2 at W = 4 keeps the pull at 4 waves
-    // per SIMD (124 VGPRs against 145 with 4 in flight: 175.2k against 165.0k
-    // MTEPS on the 1M-vertex mixed graph, 295.2k against 276.9k on R-MAT)
+    // gathers in flight (W words each): 2 at W = 4, where 4 in flight cost
+    // the pull a wave per SIMD (measured 165.0k against 175.2k MTEPS on the
+    // 1M-vertex mixed graph, 276.9k against 295.2k on R-MAT)
     constexpr int UW = W == 4 ? UNR / 4 : UNR;
     const int lane = threadIdx.x & 63;
     for (int c = a0; c < a1; c += 64) {
@@ -443,7 +460,7 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
         }
         uint64_t bal = __ballot(many<W>(f));
         while (bal) {
-            uint64_t x[UW][W];
+            SgW<SM> x[UW][W];
             int use[UW];
 #pragma unroll
             for (int u = 0; u < UW; ++u) {
@@ -453,7 +470,7 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
                 if (bal) {
                     const int src = __ffsll((long long)bal) - 1;
                     bal &= bal - 1;
-                    const int vv = __shfl(v, src, 64);
+                    const int vv = bcast(v, src);
                     const int pb = mbits<W>(mshfl<W>(f, src), lane);
                     if (pb) {
                         if (vv >= 0) {
@@ -464,7 +481,7 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
                             for (int h = 0; h < W; ++h)
                                 if (((pb >> h) & 1) && ~vv == sc[h]) {
                                     use[u] |= 1 << h;
-                                    x[u][h] = SM == SIG_F64 ? d2u(1.0) : 1ull;
+                                    x[u][h] = (SgW<SM>)(SM == SIG_F64 ? d2u(1.0) : 1ull);
                                 }
                         }
                     }
@@ -492,7 +509,7 @@ __device__ __forceinline__ void stage_flush(const BatchDev &b, int *sw, Msk<W> *
     if (cnt == 0) return;
     int base = 0;
     if (lane == 0) base = atomicAdd(&b.ctr[0], cnt);
-    base = __shfl(base, 0, 64);
+    base = bcast(base, 0);
     __builtin_amdgcn_wave_barrier();
     for (int i = lane; i < cnt; i += 64) {
         const int pos = base + i;
@@ -558,10 +575,10 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
         for (int k = k0; k < cnt; ++k) {
             const int src = __ffsll((long long)rest) - 1;
             rest &= rest - 1;
-            const int w = __shfl(w0, src, 64);
+            const int w = bcast(w0, src);
             const Msk<W> N = mshfl<W>(N0, src);
-            const int a0 = __shfl(r0, src, 64), a1 = __shfl(r1, src, 64);
-            const int ew = __shfl(e0, src, 64);
+            const int a0 = bcast(r0, src), a1 = bcast(r1, src);
+            const int ew = bcast(e0, src);
             const int pb = mbits<W>(N, lane);
             const int64_t cell = ((int64_t)w * 64 + lane) * W;
             if (!(split && is_hub(b, a1 - a0))) {
@@ -749,7 +766,7 @@ __device__ __forceinline__ void rho_sum(const BatchDev &b, const uint64_t *succ,
                 if (bal) {
                     const int src = __ffsll((long long)bal) - 1;
                     bal &= bal - 1;
-                    const int ww = __shfl(w, src, 64);
+                    const int ww = bcast(w, src);
                     use[u] = mbits<W>(mshfl<W>(x, src), lane);
                     if (use[u]) tt_row<TT, W>(b, (int64_t)ww * 64 + lane, t[u]);
                 }
@@ -843,18 +860,18 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
             if (lane >= o) inc += y;
         }
         const int pre = inc - d;
-        const int T = __shfl(inc, 63, 64);
+        const int T = bcast(inc, 63);
         int P[EB], A[EB], V[EB], EP[EB];
         Msk<W> MM[EB];
         bool OWN[EB];
 #pragma unroll
         for (int j = 0; j < EB; ++j) {
-            P[j] = j < cnt ? __shfl(pre, j, 64) : INT32_MAX;
-            A[j] = __shfl(r0, j, 64);
-            V[j] = __shfl(v0, j, 64);
-            EP[j] = __shfl(ep0, j, 64);
+            P[j] = j < cnt ? bcast(pre, j) : INT32_MAX;
+            A[j] = bcast(r0, j);
+            V[j] = bcast(v0, j);
+            EP[j] = bcast(ep0, j);
             MM[j] = mshfl<W>(M0, j);
-            OWN[j] = __shfl((int)mine, j, 64) != 0;
+            OWN[j] = bcast((int)mine, j) != 0;
 #pragma unroll
             for (int h = 0; h < W; ++h) r_rho[wv][j][h * 64 + lane] = 0.0;
         }
@@ -931,7 +948,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
 #pragma unroll
             for (int h = 0; h < W; ++h) r_rho[wv][kcur][h * 64 + lane] = acc[h].value();
         // finish every entry of the block (hubs excepted): sigma rows first
-        uint64_t SX[EB][W];
+        SgW<SM> SX[EB][W];
 #pragma unroll
         for (int j = 0; j < EB; ++j) {
 #pragma unroll
@@ -978,7 +995,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         __threadfence();
         int done = 0;
         if (lane == 0) done = atomicAdd(&b.hub_cnt[hb], 1);
-        done = __shfl(done, 0, 64);
+        done = bcast(done, 0);
         if (done != b.hub_nsl[hb] - 1) continue;
         __threadfence();
         double rho[W];
@@ -1151,7 +1168,7 @@ __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
                 accbc = i - 1 + lane < K - 1 ? b.bc[x + lane] : 0.0;
                 pv = i - 1 + lane < K - 1 ? pre[i - 1 + lane] : 0.0;
             }
-            const double p1 = __shfl(pv, (i - 1) & 63, 64);  // P(i)
+            const double p1 = bcastd(pv, (i - 1) & 63);  // P(i)
             double contrib = 0.0;
 #pragma unroll
             for (int h = 0; h < W; ++h) {
