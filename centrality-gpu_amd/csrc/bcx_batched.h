@@ -63,7 +63,10 @@ struct BatchDev {
     const int32_t *ep_idx;            // [n] endpoint index of a vertex (-1: none)
     const int32_t *ep_off;            // [n_ep + 1] chains attached to endpoint e:
     const int32_t *ep_list;           //   ep_list[ep_off[e] ..) = 2c + (0: e is L, 1: R)
+    int32_t n_ep;                     // endpoints (vertices with an attached chain)
     int32_t *ep_lvl;                  // [n_ep * 64 * W] the tile's level of each endpoint per source
+    double *ep_rho;                   // [n_ep * 64 * W] the chain terms of each endpoint's rho per
+                                      // source (made after the forward, added by the backward)
     int32_t *lane_ch;                 // [2][64 W] per source slot k + 64 h: chain of its
                                       // source (-1: a tile vertex) and its position in it
     uint64_t *lane_mask;              // [W] sources that are chain interiors
@@ -82,9 +85,11 @@ hipError_t launch_b_forward(const BatchDev &b, int sm, int l, int grid, hipStrea
 hipError_t launch_b_back(const BatchDev &b, int sm, bool fp32, int l, int grid, hipStream_t st);
 hipError_t launch_b_finish(const BatchDev &b, int grid, hipStream_t st);
 // Contracted chains: the count check between forward and backward (a tie
-// interior's count sigma_L + sigma_R must fit the tile's word), and the
-// interiors' dependencies and BC after the backward.
+// interior's count sigma_L + sigma_R must fit the tile's word), the
+// endpoints' chain terms (after the check), and the interiors' dependencies
+// and BC after the backward.
 hipError_t launch_b_chain_check(const BatchDev &b, int sm, int grid, hipStream_t st);
+hipError_t launch_b_chain_terms(const BatchDev &b, int sm, int grid, hipStream_t st);
 hipError_t launch_b_chains(const BatchDev &b, int sm, bool fp32, int grid, hipStream_t st);
 hipError_t launch_b_clear_all(const BatchDev &b, int end, int grid, hipStream_t st);
 hipError_t launch_b_fold(double *dst, double *src, const int32_t *perm, int n, int grid, hipStream_t st);

@@ -55,12 +55,15 @@ constexpr int NW = BB / 64;    // waves per block
 constexpr int UNR = 8;         // gathers in flight per lane in the sum loops (fewer for W = 4)
 constexpr int SMALL = 16;      // degree up to which one lane expands an entry
 // The level kernels wait on dependent gathers, so waves per SIMD (occupancy,
-// set by VGPRs and LDS per block) decide their speed: the staging and window
-// sizes keep both kernels' LDS at <= 13.8 KB per block (10+ blocks per CU),
-// so at <= 96 VGPRs they run 5 waves per SIMD instead of 4 (198.6k against
-// 188.9k MTEPS on the 1M-vertex mixed graph with the pull alone at 5 waves,
-// 176.1k with neither)
-constexpr int STAGE = 192, STAGE_FLUSH = 128;  // pull: list entries staged per wave, flush point
+// set by VGPRs and LDS per block) decide their speed.  The staging and window
+// sizes keep both kernels' LDS at <= 13.3 KB per block (12 blocks per CU);
+// uniform values live in scalar registers (readlane / readfirstlane), u32
+// counts in 32-bit registers, and the chain terms are made by their own
+// kernel, so with u32 counts and FP32 terms both kernels fit 80 VGPRs: 6
+// waves per SIMD.  On the 1M-vertex mixed graph: 176.1k MTEPS at 4 waves,
+// 198.6k at 5 (188.9k with the pull alone at 5), 221.7k at 6 (215.7k with
+// the same registers at 5).
+constexpr int STAGE = 176, STAGE_FLUSH = 112;  // pull: list entries staged per wave, flush point
 constexpr int FEB = 8;         // list entries per wave block in the backward
 constexpr int WIN = 64;        // arcs per compaction window of a backward block
 
@@ -78,6 +81,13 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ double bcastd(double x, int src) { return u2d(shfl64(d2u(x), src)); }
+// a wave-uniform value (e.g. read from LDS at a uniform address) moved into
+// scalar registers
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 __device__ __forceinline__ uint64_t *mbuf(const BatchDev &b, int l) {
     const int r = l % 3;
@@ -443,7 +453,7 @@ __global__ __launch_bounds__(BB) void k_b_expand(BatchDev b, int l) {
 // (sc[h] == c); cm holds the tile's chain sources.
 template <int SM, int W>
 __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr, const Msk<W> &N, int a0, int a1,
-                                          uint64_t (&acc)[W], double (&facc)[W], bool &ovf, const int (&sc)[W],
+                                          SgW<SM> (&acc)[W], double (&facc)[W], bool &ovf, const int (&sc)[W],
                                           const Msk<W> &cm) {
     // gathers in flight (W words each): 2 at W = 4, where 4 in flight cost
     // the pull a wave per SIMD (measured 165.0k against 175.2k MTEPS on the
@@ -495,7 +505,7 @@ __device__ __forceinline__ void sigma_sum(const BatchDev &b, const uint64_t *fr,
                     if (SM == SIG_F64) {
                         facc[h] += u2d(x[u][h]);
                     } else {
-                        const uint64_t s = acc[h] + x[u][h];
+                        const SgW<SM> s = acc[h] + x[u][h];  // (wrap: the word overflowed)
                         ovf |= (s < acc[h]);
                         acc[h] = s;
                     }
@@ -575,14 +585,18 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
         for (int k = k0; k < cnt; ++k) {
             const int src = __ffsll((long long)rest) - 1;
             rest &= rest - 1;
-            const int w = bcast(w0, src);
-            const Msk<W> N = mshfl<W>(N0, src);
+            // (the entry's vertex and mask from the staging slots, so the
+            // chunk's masks need no registers through the entry loop)
+            const int w = __builtin_amdgcn_readfirstlane(s_w[wv][k]);
+            Msk<W> N;
+#pragma unroll
+            for (int h = 0; h < W; ++h) N.m[h] = uni64(s_m[wv][k].m[h]);
             const int a0 = bcast(r0, src), a1 = bcast(r1, src);
             const int ew = bcast(e0, src);
             const int pb = mbits<W>(N, lane);
             const int64_t cell = ((int64_t)w * 64 + lane) * W;
             if (!(split && is_hub(b, a1 - a0))) {
-                uint64_t acc[W];
+                SgW<SM> acc[W];
                 double facc[W];
                 int sc[W];
 #pragma unroll
@@ -622,7 +636,8 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
         if (b.sl_j0[s] == b.rp[v] && lane == 0 && l + 1 < b.hub_stride)
             mst<W>(b.hub_mask, (int64_t)hb * b.hub_stride + l + 1, N);
         if (!split || !many<W>(N)) continue;
-        uint64_t acc[W];
+        constexpr int SI = F64 ? SIG_U64 : SM;  // (not reached in FP64 mode)
+        SgW<SI> acc[W];
         double facc[W];
         int sc[W];
 #pragma unroll
@@ -632,7 +647,6 @@ __global__ __launch_bounds__(BB) void k_b_pull(BatchDev b, int l) {
             sc[h] = (v == mL[h] || v == mR[h]) ? myc[h] : -1;
         }
         bool ovf = false;
-        constexpr int SI = F64 ? SIG_U64 : SM;  // (not reached in FP64 mode)
         sigma_sum<SI, W>(b, fr, N, b.sl_j0[s], b.sl_j1[s], acc, facc, ovf, sc, cm);
         const int pb = mbits<W>(N, lane);
 #pragma unroll
@@ -795,7 +809,7 @@ __device__ __forceinline__ void finish_vertex(const BatchDev &b, int v, int l, c
         const int64_t cell = ((int64_t)v * 64 + lane) * W + h;
         const double sv = sg_val<SM>(sg<SM>(b, cell));
         double a = acc[h];
-        if (e >= 0) a += chain_rho<SM, W>(b, e, l, sv, lane, h);
+        if (e >= 0) a += b.ep_rho[((int64_t)e * 64 + lane) * W + h];
         const double delta = sv * a;
         contrib += b.wsl[h * 64 + lane] * delta;
         ((TT *)b.tt)[cell] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
@@ -921,7 +935,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
 #pragma unroll
                     for (int h = 0; h < W; ++h) t[u][h] = 0;
                     if (rr < nrel) {
-                        kk[u] = t_k[wv][rr];
+                        kk[u] = __builtin_amdgcn_readfirstlane(t_k[wv][rr]);  // (uniform: a scalar register)
                         use[u] = mbits<W>(t_x[wv][rr], lane);
                         if (use[u]) tt_row<TT, W>(b, (int64_t)t_w[wv][rr] * 64 + lane, t[u]);
                     }
@@ -947,6 +961,15 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
         if (kcur >= 0)
 #pragma unroll
             for (int h = 0; h < W; ++h) r_rho[wv][kcur][h * 64 + lane] = acc[h].value();
+        // chain endpoints: their chains' terms join rho
+#pragma unroll
+        for (int j = 0; j < EB; ++j) {
+            if (!OWN[j] || EP[j] < 0) continue;
+            const int pb = mbits<W>(MM[j], lane);
+#pragma unroll
+            for (int h = 0; h < W; ++h)
+                if ((pb >> h) & 1) r_rho[wv][j][h * 64 + lane] += b.ep_rho[((int64_t)EP[j] * 64 + lane) * W + h];
+        }
         // finish every entry of the block (hubs excepted): sigma rows first
         SgW<SM> SX[EB][W];
 #pragma unroll
@@ -966,8 +989,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
                 if (!((pb >> h) & 1)) continue;
                 const int64_t cell = ((int64_t)v * 64 + lane) * W + h;
                 const double sv = sg_val<SM>(SX[j][h]);
-                double a = r_rho[wv][j][h * 64 + lane];  // rho_v
-                if (EP[j] >= 0) a += chain_rho<SM, W>(b, EP[j], l, sv, lane, h);
+                const double a = r_rho[wv][j][h * 64 + lane];  // rho_v
                 const double delta = sv * a;
                 contrib += b.wsl[h * 64 + lane] * delta;
                 ((TT *)b.tt)[cell] = (TT)(((double)b.reach[v] + 1.0) / sv + a);
@@ -1077,6 +1099,30 @@ __global__ __launch_bounds__(BB) void k_b_chain_check(BatchDev b) {
                 if (tie(aL + myj, myj)) check(wL, 1);
                 if (tie(aR + K - myj, K - myj)) check(wR, 1);
             }
+        }
+    }
+}
+
+// After the forward and the count check: the chain terms of every endpoint's
+// rho for every source, which depend on the endpoints' levels and counts
+// alone, into ep_rho; the backward adds them when it finishes an endpoint
+// (inlined there, the closed forms raised the backward's register peak by a
+// wave per SIMD).  One wave per endpoint.
+template <int SM, int W>
+__global__ __launch_bounds__(BB) void k_b_chain_terms(BatchDev b) {
+    constexpr bool F64 = SM == SIG_F64;
+    if (ldi(&b.ctr[3]) < 0 || (!F64 && b.ctr[1]) || b.ctr[2]) return;
+    const int lane = threadIdx.x & 63;
+    const int ns = b.ctr[5];
+    for (int e = gwave(); e < b.n_ep; e += nwaves()) {
+        const int code = b.ep_list[b.ep_off[e]];  // (an endpoint has a chain: its vertex is that chain's end)
+        const int v = (code & 1) ? b.ch_R[code >> 1] : b.ch_L[code >> 1];
+#pragma unroll 1
+        for (int h = 0; h < W; ++h) {
+            if (h * 64 + lane >= ns) continue;
+            const int64_t c = ((int64_t)e * 64 + lane) * W + h;
+            const double sv = sg_val<SM>(sg<SM>(b, ((int64_t)v * 64 + lane) * W + h));
+            b.ep_rho[c] = chain_rho<SM, W>(b, e, b.ep_lvl[c], sv, lane, h);
         }
     }
 }
@@ -1326,6 +1372,23 @@ hipError_t launch_b_chain_check(const BatchDev &b, int sm, int grid, hipStream_t
     if (b.spl == 4) chain_check_w<4>(b, sm, chain_grid(b), st);
     else if (b.spl == 2) chain_check_w<2>(b, sm, chain_grid(b), st);
     else chain_check_w<1>(b, sm, chain_grid(b), st);
+    return hipGetLastError();
+}
+
+template <int W>
+static void chain_terms_w(const BatchDev &b, int sm, int g, hipStream_t st) {
+    if (sm == SIG_U32) hipLaunchKernelGGL((k_b_chain_terms<SIG_U32, W>), dim3(g), dim3(BB), 0, st, b);
+    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_chain_terms<SIG_U64, W>), dim3(g), dim3(BB), 0, st, b);
+    else hipLaunchKernelGGL((k_b_chain_terms<SIG_F64, W>), dim3(g), dim3(BB), 0, st, b);
+}
+
+// one wave per endpoint
+hipError_t launch_b_chain_terms(const BatchDev &b, int sm, int grid, hipStream_t st) {
+    (void)grid;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)b.n_ep + NW - 1) / NW, 1 << 16));
+    if (b.spl == 4) chain_terms_w<4>(b, sm, g, st);
+    else if (b.spl == 2) chain_terms_w<2>(b, sm, g, st);
+    else chain_terms_w<1>(b, sm, g, st);
     return hipGetLastError();
 }
 

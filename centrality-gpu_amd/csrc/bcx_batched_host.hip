@@ -51,7 +51,7 @@ constexpr int GRAPHS_PER_SLOT = 12;
 
 struct Slot {
     DBuf vis, mb, sig, tt, Q, Qm, ctr, qoff, hub_mask, hub_acc, hub_cnt, lvl, dbg, bcacc, wsl;
-    DBuf eplv, lanes;  // contracted chains: endpoint levels per lane, lane table
+    DBuf eplv, eprho, lanes;  // contracted chains: endpoint levels and chain terms per lane, lane table
     int32_t n_cap = 0, qcap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -221,6 +221,8 @@ static int launch_levels(bcx_ctx *ctx, Slot &s, const BatchDev &b, int L0, int C
     if (b.n_chain > 0 && s.sm != SIG_F64 &&
         (rc = ctx_hip(ctx, launch_b_chain_check(b, s.sm, j.grid, s.stream), "chain check")))
         return rc;
+    if (b.n_chain > 0 && (rc = ctx_hip(ctx, launch_b_chain_terms(b, s.sm, j.grid, s.stream), "chain terms")))
+        return rc;
     for (int l = L0 + C - 1; l >= 1; --l)
         if ((rc = ctx_hip(ctx, launch_b_back(b, s.sm, j.fp32, l, j.grid, s.stream), "back"))) return rc;
     if (b.n_chain > 0 && (rc = ctx_hip(ctx, launch_b_chains(b, s.sm, j.fp32, j.grid, s.stream), "chains")))
@@ -282,7 +284,7 @@ static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st)
         return rc;
     }
     s.L += s.C;
-    if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1 + (b.n_chain > 0 ? 1 + (s.sm != SIG_F64) : 0);
+    if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1 + (b.n_chain > 0 ? 2 + (s.sm != SIG_F64) : 0);
     if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.pinned, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream),
                       "counters")))
         return rc;
@@ -1110,7 +1112,9 @@ static void set_chains(BatchDev &b, Slot &s, const ChainCut *cut) {
     b.ep_idx = cut->ep_idx;
     b.ep_off = cut->ep_off;
     b.ep_list = cut->ep_list;
+    b.n_ep = cut->n_ep;
     b.ep_lvl = (int32_t *)s.eplv.p;
+    b.ep_rho = (double *)s.eprho.p;
     b.lane_ch = (int32_t *)s.lanes.p;
     b.lane_mask = (uint64_t *)((char *)s.lanes.p + 2 * 64 * MAX_SPL * sizeof(int32_t));
 }
@@ -1122,6 +1126,7 @@ static int cut_slots(bcx_ctx *ctx, const ChainCut *cut) {
     for (int i = 0; i < bs.n_slots; ++i) {
         Slot &s = bs.slots[i];
         if ((rc = s.eplv.ensure(ctx, 4 * 64 * MAX_SPL * (size_t)std::max(cut->n_ep, 1), "endpoint levels")) ||
+            (rc = s.eprho.ensure(ctx, 8 * 64 * MAX_SPL * (size_t)std::max(cut->n_ep, 1), "endpoint chain terms")) ||
             (rc = s.lanes.ensure(ctx, (2 * 64 * sizeof(int32_t) + sizeof(uint64_t)) * MAX_SPL, "lane chains")))
             return rc;
     }
@@ -1759,7 +1764,7 @@ void batched_release(bcx_ctx *ctx) {
         for (TileGraph &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
         s.graphs.clear();
         for (DBuf *b : {&s.vis, &s.mb, &s.sig, &s.tt, &s.Q, &s.Qm, &s.ctr, &s.qoff, &s.hub_mask, &s.hub_acc,
-                        &s.hub_cnt, &s.lvl, &s.dbg, &s.bcacc, &s.wsl, &s.eplv, &s.lanes})
+                        &s.hub_cnt, &s.lvl, &s.dbg, &s.bcacc, &s.wsl, &s.eplv, &s.eprho, &s.lanes})
             b->release(ctx);
         if (s.pinned) (void)hipHostFree(s.pinned);
         if (s.done) (void)hipEventDestroy(s.done);
