@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--summarize-trace", default="",
                     help="rocprofv3 kernel-trace CSV: print per-kernel calls, summed and union busy time")
     ap.add_argument("--slots", type=int, default=0, help="concurrent tiles of the batched path (0 = library default)")
+    ap.add_argument("--bpc", type=int, default=0,
+                    help="level-kernel blocks per CU of the multi-launch tiles (0 = tuned)")
     ap.add_argument("--spl", type=int, default=0,
                     help="sources per lane of the multi-launch tiles (1, 2, 4; 0 = tuned), for A/B measurements")
     ap.add_argument("--flags", type=int, default=0,
@@ -238,7 +240,7 @@ def main():
     idx = [i % args.slices for i in range(K)]
     t_b, tk_b, st_b = timed_steps(g, slices, idx, args.warmup, path=bcx.PATH_BATCHED,
                                   precision=bcx.PREC_FP32C, use_chains=True, batch_width=args.slots,
-                                  flags=args.flags, sources_per_lane=args.spl)
+                                  flags=args.flags, sources_per_lane=args.spl, blocks_per_cu=args.bpc)
     src_b = sum(len(slices[i][0]) for i in idx)
     work_b = W * src_b / total_src
     mteps_b = work_b / t_b / 1e6

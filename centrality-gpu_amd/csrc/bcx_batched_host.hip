@@ -1535,15 +1535,15 @@ int batched_kind(DComp &dc, int flags) {
 constexpr int32_t TUNE_N = 16384;
 constexpr int TUNE_TILES = 24;
 
-// Sources of k tiles evenly spaced over the component's tile order.
-static std::vector<int32_t> sample_tiles(DComp &dc, int64_t k) {
+// Sources of k tiles of `width` evenly spaced over the component's tile order.
+static std::vector<int32_t> sample_tiles(DComp &dc, int64_t k, int64_t width) {
     const std::vector<int32_t> order = batched_order(dc);
-    const int64_t T = ((int64_t)dc.n + BATCH_W - 1) / BATCH_W;
+    const int64_t T = ((int64_t)dc.n + width - 1) / width;
     k = std::min(k, T);
     std::vector<int32_t> out;
     for (int64_t i = 0; i < k; ++i) {
         const int64_t t = (2 * i + 1) * T / (2 * k);
-        for (int64_t v = t * BATCH_W; v < std::min<int64_t>((t + 1) * BATCH_W, dc.n); ++v)
+        for (int64_t v = t * width; v < std::min<int64_t>((t + 1) * width, dc.n); ++v)
             out.push_back(order[(size_t)v]);
     }
     return out;
@@ -1590,12 +1590,17 @@ int batched_schedule(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, bcx_stats *st) 
         return r;
     };
     const MultiSched base = multi_sched(dc, o);  // the defaults, or what the options fix
-    const std::vector<int32_t> samp = rc ? std::vector<int32_t>() : sample_tiles(dc, TUNE_TILES);
+    // every multi-launch candidate runs the same sources: TUNE_TILES tiles of
+    // the widest lanes, which narrower lanes cut into more tiles (a sample of
+    // 64-source tiles left the wide candidates a few tiles for eight slots and
+    // picked 128-source tiles where 256-source ones run 10% faster)
+    const std::vector<int32_t> samp =
+        rc ? std::vector<int32_t>() : sample_tiles(dc, TUNE_TILES, (int64_t)BATCH_W * MAX_SPL);
     double r_multi = 0.0;
     if (!rc) rc = time_multi(samp, base, &r_multi);
     if (!rc && need_kind) {
         double r_wg = 0.0;
-        rc = time_wg(sample_tiles(dc, std::max(1, ctx->n_cu)), &r_wg);
+        rc = time_wg(sample_tiles(dc, std::max(1, ctx->n_cu), BATCH_W), &r_wg);
         kind = r_wg >= r_multi ? BCX_KERNEL_WG : BCX_KERNEL_MULTI;
     }
     MultiSched best = base;
