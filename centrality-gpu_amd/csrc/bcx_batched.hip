@@ -1127,82 +1127,62 @@ __global__ __launch_bounds__(BB) void k_b_chain_terms(BatchDev b) {
     }
 }
 
-// After the backward: every contracted interior of every chain, one wave per
-// chain (the launch has a wave for every chain, so the chains' dependent
-// loads overlap) and one lane per source slot; BC into the slot's
-// accumulator (each interior has one writer), and in the probe the
-// interiors' rows.  For a source off the chain, the segment L .. R's
-// constants (split point, tie, the prefix sums at the split) are set once
-// per chain, so an interior costs the chain's running prefix sum (the same
-// word for every lane), a few selects and the wave sum; the rare source that
-// lies on the chain takes the general form for its two halves.
-template <int SM, typename TT, int W>
+// After the backward: the BC of every contracted interior from the sources
+// off its chain, one wave per chain (the launch has a wave for every chain,
+// so the chains' dependent loads overlap) and one lane per source slot; BC
+// into the slot's accumulator (each interior has one writer).  For such a
+// source the segment L .. R has a split mA and possibly a tie interior t, and
+// an interior's dependency is cA - P(i) on L's side, 0 at the tie and
+// P(i-1) - cB on R's side, with cA, cB set once per chain; an interior then
+// costs the chain's running prefix sum (the same word for every lane, loaded
+// coalesced once per 64 interiors), a few selects and the wave sum.  The
+// sources that lie on a chain (k_b_chains_own) and the probe's rows
+// (k_b_chain_rows) have kernels of their own, so this one stays small in
+// registers and shares the compute units with the other tiles' level kernels
+// (with them inlined it took every register of its SIMD).
+template <int SM, int W>
 __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
     constexpr bool F64 = SM == SIG_F64;
     if (ldi(&b.ctr[3]) < 0 || (!F64 && b.ctr[1]) || b.ctr[2]) return;
     const int lane = threadIdx.x & 63;
     const int ns = b.ctr[5];
-    const uint64_t one = F64 ? d2u(1.0) : 1ull;
-    const TT *tt = (const TT *)b.tt;
-    bool act[W];
-    double wgt[W];
-    int myc[W], myj[W];
-#pragma unroll
-    for (int h = 0; h < W; ++h) {
-        const int i = h * 64 + lane;
-        act[h] = i < ns;
-        wgt[h] = b.wsl[i];
-        myc[h] = b.lane_ch[i];
-        myj[h] = b.lane_ch[64 * W + i];
-    }
     for (int c = gwave(); c < b.n_chain; c += nwaves()) {
         const int L = b.ch_L[c], R = b.ch_R[c], off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
         const double *pre = b.ch_pre + off;
         const ChainPfx P{pre, 0, 1};
         const int64_t eL = (int64_t)b.ep_idx[L] * 64, eR = (int64_t)b.ep_idx[R] * 64;
-        int aL[W], aR[W], t[W], mA[W], b0[W];
-        bool tie[W], own[W];
-        uint64_t wL[W], wR[W], wT[W];
-        double sL[W], sR[W], tL[W], tR[W], cA[W], cB[W];
-        bool any_own = false;
+        int mA[W], tq[W];
+        double wgt[W], cA[W], cB[W];
 #pragma unroll
         for (int h = 0; h < W; ++h) {
-            aL[h] = aR[h] = 0;
-            wL[h] = wR[h] = 0;
-            sL[h] = sR[h] = 1.0;
-            tL[h] = tR[h] = 0.0;
-            if (act[h]) {
-                aL[h] = b.ep_lvl[(eL + lane) * W + h];
-                aR[h] = b.ep_lvl[(eR + lane) * W + h];
-                wL[h] = sg<SM>(b, ((int64_t)L * 64 + lane) * W + h);
-                wR[h] = sg<SM>(b, ((int64_t)R * 64 + lane) * W + h);
-                sL[h] = sg_val<SM>(wL[h]);
-                sR[h] = sg_val<SM>(wR[h]);
-                if (c == myc[h]) {
-                    tL[h] = (double)tt[((int64_t)L * 64 + lane) * W + h];
-                    tR[h] = (double)tt[((int64_t)R * 64 + lane) * W + h];
-                }
-            }
-            own[h] = act[h] && c == myc[h];
-            any_own |= __ballot(own[h]) != 0;
+            const int i = h * 64 + lane;
+            mA[h] = 0;
+            tq[h] = -1;
+            wgt[h] = cA[h] = cB[h] = 0.0;  // (weight 0: the slot adds nothing)
+            if (i >= ns || b.lane_ch[i] == c) continue;
+            const int aL = b.ep_lvl[(eL + lane) * W + h], aR = b.ep_lvl[(eR + lane) * W + h];
+            const double sL = sg_val<SM>(sg<SM>(b, ((int64_t)L * 64 + lane) * W + h));
+            const double sR = sg_val<SM>(sg<SM>(b, ((int64_t)R * 64 + lane) * W + h));
             // segment L .. R seen from L: split, tie and the prefix sums there
-            const int S = aR[h] - aL[h] + K;
-            t[h] = -1;
-            tie[h] = false;
+            const int S = aR - aL + K;
+            int t = -1, m, b0;
+            bool tie = false;
             if ((S & 1) == 0) {
-                t[h] = S >> 1;
-                mA[h] = t[h] - 1;
-                tie[h] = t[h] >= 1 && t[h] <= K - 1;
-                b0[h] = t[h] + 1;
+                t = S >> 1;
+                m = t - 1;
+                tie = t >= 1 && t <= K - 1;
+                b0 = t + 1;
             } else {
-                mA[h] = (S - 1) >> 1;
-                b0[h] = mA[h] + 1;
+                m = (S - 1) >> 1;
+                b0 = m + 1;
             }
-            mA[h] = min(mA[h], K - 1);
-            const double tw = tie[h] ? P(t[h]) - P(t[h] - 1) : 0.0;
-            cA[h] = P(mA[h]) + (tie[h] ? tw * sL[h] / (sL[h] + sR[h]) : 0.0);
-            cB[h] = P(min(b0[h], K) - 1) - (tie[h] ? tw * sR[h] / (sL[h] + sR[h]) : 0.0);
-            wT[h] = F64 ? d2u(sL[h] + sR[h]) : wL[h] + wR[h];
+            m = min(m, K - 1);
+            const double tw = tie ? P(t) - P(t - 1) : 0.0;
+            mA[h] = m;
+            tq[h] = tie ? t : -1;
+            wgt[h] = b.wsl[i];
+            cA[h] = P(m) + (tie ? tw * sL / (sL + sR) : 0.0);
+            cB[h] = P(min(b0, K) - 1) - (tie ? tw * sR / (sL + sR) : 0.0);
         }
         double p0 = 0.0;  // P(i - 1)
         // the interiors' BC words and prefix sums, 64 at a time, one per lane
@@ -1217,50 +1197,102 @@ __global__ __launch_bounds__(BB) void k_b_chains(BatchDev b) {
             const double p1 = bcastd(pv, (i - 1) & 63);  // P(i)
             double contrib = 0.0;
 #pragma unroll
-            for (int h = 0; h < W; ++h) {
-                int lev;
-                uint64_t sw;
-                double dl;
-                if (i <= mA[h]) {
-                    lev = aL[h] + i;
-                    sw = wL[h];
-                    dl = cA[h] - p1;
-                } else if (tie[h] && i == t[h]) {
-                    lev = aL[h] + t[h];
-                    sw = wT[h];
-                    dl = 0.0;
-                } else {
-                    lev = aR[h] + K - i;
-                    sw = wR[h];
-                    dl = p0 - cB[h];
-                }
-                if (any_own && own[h]) {
-                    lev = 0;
-                    sw = one;
-                    dl = 0.0;
-                    const int j = myj[h];
-                    if (i < j)  // the half from the source to L, numbered from the source
-                        seg_interior<SM>(0, aL[h], 1.0, sL[h], one, wL[h], j, j - i, ChainPfx{pre, j - 1, -1},
-                                         tL[h], lev, sw, dl);
-                    else if (i > j)  // the half from the source to R
-                        seg_interior<SM>(0, aR[h], 1.0, sR[h], one, wR[h], K - j, i - j, ChainPfx{pre, j, 1},
-                                         tR[h], lev, sw, dl);
-                }
-                if (act[h]) {
-                    contrib += wgt[h] * dl;
-                    if (b.lvl) {
-                        const int64_t cell = (x * 64 + lane) * W + h;
-                        b.lvl[cell] = lev;
-                        sg_put<SM>(b, cell, sw);
-                        b.dbg_delta[cell] = dl;
-                    }
-                }
-            }
+            for (int h = 0; h < W; ++h)
+                contrib += wgt[h] * (i <= mA[h] ? cA[h] - p1 : (i == tq[h] ? 0.0 : p0 - cB[h]));
             p0 = p1;
             contrib = wave_sum(contrib);
             if (lane == ((i - 1) & 63)) accbc += contrib;
             if ((((i - 1) & 63) == 63 || i == K - 1) && lane <= ((i - 1) & 63))
                 b.bc[(int64_t)b.n + off + ((i - 1) & ~63) + lane] = accbc;
+        }
+    }
+}
+
+// The end of a source's own chain segment: its level, count, dependency (and
+// the probe's row) at interior x of the chain c holding source x_j: the half
+// from the source to L (x < j, numbered from the source) or to R (x > j), each
+// a segment from the source (level 0, count 1) to the endpoint.
+template <int SM>
+__device__ __forceinline__ void own_interior(int x, int j, int K, const double *pre, int aL, int aR, double sL,
+                                             double sR, uint64_t wL, uint64_t wR, double tL, double tR, int &lev,
+                                             uint64_t &sw, double &dl) {
+    const uint64_t one = SM == SIG_F64 ? d2u(1.0) : 1ull;
+    lev = 0;
+    sw = one;
+    dl = 0.0;
+    if (x < j)
+        seg_interior<SM>(0, aL, 1.0, sL, one, wL, j, j - x, ChainPfx{pre, j - 1, -1}, tL, lev, sw, dl);
+    else if (x > j)
+        seg_interior<SM>(0, aR, 1.0, sR, one, wR, K - j, x - j, ChainPfx{pre, j, 1}, tR, lev, sw, dl);
+}
+
+// After k_b_chains: the interiors' BC from the sources that lie on a chain,
+// one wave per source slot (the launch has a wave for each; slots of tile
+// vertices return at once) and one lane per interior of the source's chain,
+// added with atomics (several sources can lie on one chain).
+template <int SM, typename TT, int W>
+__global__ __launch_bounds__(BB) void k_b_chains_own(BatchDev b) {
+    constexpr bool F64 = SM == SIG_F64;
+    if (ldi(&b.ctr[3]) < 0 || (!F64 && b.ctr[1]) || b.ctr[2]) return;
+    const int i = gwave();  // source slot k + 64 h
+    if (i >= min(b.ctr[5], 64 * W)) return;
+    const int c = b.lane_ch[i];
+    if (c < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int j = b.lane_ch[64 * W + i], k = i & 63, h = i >> 6;
+    const int L = b.ch_L[c], R = b.ch_R[c], off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
+    const int64_t cL = ((int64_t)L * 64 + k) * W + h, cR = ((int64_t)R * 64 + k) * W + h;
+    const int aL = b.ep_lvl[((int64_t)b.ep_idx[L] * 64 + k) * W + h];
+    const int aR = b.ep_lvl[((int64_t)b.ep_idx[R] * 64 + k) * W + h];
+    const uint64_t wL = sg<SM>(b, cL), wR = sg<SM>(b, cR);
+    const double tL = (double)((const TT *)b.tt)[cL], tR = (double)((const TT *)b.tt)[cR];
+    const double wgt = b.wsl[i];
+    for (int x = 1 + lane; x < K; x += 64) {
+        int lev;
+        uint64_t sw;
+        double dl;
+        own_interior<SM>(x, j, K, b.ch_pre + off, aL, aR, sg_val<SM>(wL), sg_val<SM>(wR), wL, wR, tL, tR, lev, sw,
+                         dl);
+        if (dl != 0.0) unsafeAtomicAdd(&b.bc[(int64_t)b.n + off + x - 1], wgt * dl);
+    }
+}
+
+// Probe only: the level, count word and dependency of every (interior,
+// source) pair, one wave per chain and one lane per source slot.
+template <int SM, typename TT, int W>
+__global__ __launch_bounds__(BB) void k_b_chain_rows(BatchDev b) {
+    constexpr bool F64 = SM == SIG_F64;
+    if (ldi(&b.ctr[3]) < 0 || (!F64 && b.ctr[1]) || b.ctr[2]) return;
+    const int lane = threadIdx.x & 63;
+    const int ns = b.ctr[5];
+    for (int c = gwave(); c < b.n_chain; c += nwaves()) {
+        const int L = b.ch_L[c], R = b.ch_R[c], off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
+        const double *pre = b.ch_pre + off;
+        const int64_t eL = (int64_t)b.ep_idx[L] * 64, eR = (int64_t)b.ep_idx[R] * 64;
+        for (int h = 0; h < W; ++h) {
+            const int i = h * 64 + lane;
+            if (i >= ns) continue;
+            const int64_t cL = ((int64_t)L * 64 + lane) * W + h, cR = ((int64_t)R * 64 + lane) * W + h;
+            const int aL = b.ep_lvl[(eL + lane) * W + h], aR = b.ep_lvl[(eR + lane) * W + h];
+            const uint64_t wL = sg<SM>(b, cL), wR = sg<SM>(b, cR);
+            const double sL = sg_val<SM>(wL), sR = sg_val<SM>(wR);
+            const bool own = b.lane_ch[i] == c;
+            const int j = b.lane_ch[64 * W + i];
+            const double tL = own ? (double)((const TT *)b.tt)[cL] : 0.0;
+            const double tR = own ? (double)((const TT *)b.tt)[cR] : 0.0;
+            for (int x = 1; x < K; ++x) {
+                int lev;
+                uint64_t sw;
+                double dl;
+                if (own)
+                    own_interior<SM>(x, j, K, pre, aL, aR, sL, sR, wL, wR, tL, tR, lev, sw, dl);
+                else
+                    seg_interior<SM>(aL, aR, sL, sR, wL, wR, K, x, ChainPfx{pre, 0, 1}, 0.0, lev, sw, dl);
+                const int64_t cell = (((int64_t)b.n + off + x - 1) * 64 + lane) * W + h;
+                b.lvl[cell] = lev;
+                sg_put<SM>(b, cell, sw);
+                b.dbg_delta[cell] = dl;
+            }
         }
     }
 }
@@ -1392,15 +1424,24 @@ hipError_t launch_b_chain_terms(const BatchDev &b, int sm, int grid, hipStream_t
     return hipGetLastError();
 }
 
-template <int W>
-static void chains_w(const BatchDev &b, int sm, bool fp32, int g, hipStream_t st) {
-    if (sm == SIG_U32 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U32, float, W>), dim3(g), dim3(BB), 0, st, b);
-    else if (sm == SIG_U32) hipLaunchKernelGGL((k_b_chains<SIG_U32, double, W>), dim3(g), dim3(BB), 0, st, b);
-    else if (sm == SIG_U64 && fp32) hipLaunchKernelGGL((k_b_chains<SIG_U64, float, W>), dim3(g), dim3(BB), 0, st, b);
-    else if (sm == SIG_U64) hipLaunchKernelGGL((k_b_chains<SIG_U64, double, W>), dim3(g), dim3(BB), 0, st, b);
-    else hipLaunchKernelGGL((k_b_chains<SIG_F64, double, W>), dim3(g), dim3(BB), 0, st, b);
+template <int SM, typename TT, int W>
+static void chains_k(const BatchDev &b, int g, hipStream_t st) {
+    hipLaunchKernelGGL((k_b_chains<SM, W>), dim3(g), dim3(BB), 0, st, b);
+    hipLaunchKernelGGL((k_b_chains_own<SM, TT, W>), dim3(64 * W / NW), dim3(BB), 0, st, b);
+    if (b.lvl) hipLaunchKernelGGL((k_b_chain_rows<SM, TT, W>), dim3(g), dim3(BB), 0, st, b);
 }
 
+template <int W>
+static void chains_w(const BatchDev &b, int sm, bool fp32, int g, hipStream_t st) {
+    if (sm == SIG_U32 && fp32) chains_k<SIG_U32, float, W>(b, g, st);
+    else if (sm == SIG_U32) chains_k<SIG_U32, double, W>(b, g, st);
+    else if (sm == SIG_U64 && fp32) chains_k<SIG_U64, float, W>(b, g, st);
+    else if (sm == SIG_U64) chains_k<SIG_U64, double, W>(b, g, st);
+    else chains_k<SIG_F64, double, W>(b, g, st);
+}
+
+// the interiors' BC (and, probing, their rows): one wave per chain, then one
+// per source slot for the sources on chains
 hipError_t launch_b_chains(const BatchDev &b, int sm, bool fp32, int grid, hipStream_t st) {
     (void)grid;
     if (b.spl == 4) chains_w<4>(b, sm, fp32, chain_grid(b), st);

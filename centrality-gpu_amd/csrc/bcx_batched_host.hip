@@ -284,7 +284,7 @@ static int enqueue_chunk(bcx_ctx *ctx, Slot &s, const TileJob &j, bcx_stats *st)
         return rc;
     }
     s.L += s.C;
-    if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1 + (b.n_chain > 0 ? 2 + (s.sm != SIG_F64) : 0);
+    if (st) st->n_launches += 2 * s.C + (s.L - 1) + 1 + (b.n_chain > 0 ? 3 + (s.sm != SIG_F64) + (b.lvl != nullptr) : 0);
     if ((rc = ctx_hip(ctx, hipMemcpyAsync(s.pinned, b.ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream),
                       "counters")))
         return rc;
