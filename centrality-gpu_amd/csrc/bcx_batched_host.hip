@@ -1183,6 +1183,14 @@ constexpr int DEFAULT_SPL = 4;
 
 static bool valid_spl(int w) { return w == 1 || w == 2 || w == 4; }
 
+// Sources per lane of the workgroup kernel's tiles (bcx_opts.sources_per_lane
+// when valid).  1: on the 256x256 grid workload, tiles of 128 and 256 sources
+// ran 6.6k and 1.5k MTEPS against 10.2k for 64 (a deep tile's work grows with
+// its width faster than its levels are shared, and 256-source tiles' lists
+// leave room for fewer workgroups).
+constexpr int WG_DEFAULT_SPL = 1;
+static int wg_spl(const bcx_opts *o) { return valid_spl(o->sources_per_lane) ? o->sources_per_lane : WG_DEFAULT_SPL; }
+
 static MultiSched multi_sched(const DComp &dc, const bcx_opts *o) {
     MultiSched m;
     m.cut = want_cut(o) && dc.sched_cut != 0;
@@ -1233,7 +1241,7 @@ void batched_sched_report(const DComp &dc, const bcx_opts *o, int32_t *bpc, int3
 // small components: one persistent launch over (component, 64-source) tiles
 // ------------------------------------------------------------------------
 struct SmallState {
-    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs, qr, ro, lvl, dbg, psig;
+    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs, qr, ro, tag, lvl, dbg, psig;
     int grid = 0;
     // a pass started and not yet finished (persistent_start / _finish)
     bool pending = false, run_fp32 = false, run_graph = false;
@@ -1330,11 +1338,14 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     for (const SmallJob &jb : jobs) order.push_back(&jb);
     auto wide = [&](const SmallJob *jb) { return wg && jb->dc->sigma_est > 4.6e18; };
     std::stable_partition(order.begin(), order.end(), [&](const SmallJob *jb) { return !wide(jb); });
+    // tiles of 64 W sources (source k + 64 h on lane k); the small kernel's are 64
+    const int W = wg ? wg_spl(o) : 1;
+    const size_t TW = (size_t)BATCH_W * W;
     size_t n_narrow_tiles = 0;
     for (const SmallJob *jbp : order) {
         const SmallJob &jb = *jbp;
         const DComp &dc = *jb.dc;
-        if (!wide(jbp)) n_narrow_tiles += (jb.srcs.size() + BATCH_W - 1) / BATCH_W;
+        if (!wide(jbp)) n_narrow_tiles += (jb.srcs.size() + TW - 1) / TW;
         // (the workgroup kernel stays on the component's own ids: on the
         // BFS-ordered copy the 256x256 grid workload fetched 32% more bytes
         // and ran 8% slower)
@@ -1346,11 +1357,11 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
         c.bc = bc_base + dc.offset;
         const int32_t ci = (int32_t)comps.size();
         comps.push_back(c);
-        for (size_t i = 0; i < jb.srcs.size(); i += BATCH_W) {
+        for (size_t i = 0; i < jb.srcs.size(); i += TW) {
             SmallTile t{};
             t.comp = ci;
             t.src0 = (int32_t)srcs.size();
-            t.ns = (int32_t)std::min<size_t>(BATCH_W, jb.srcs.size() - i);
+            t.ns = (int32_t)std::min<size_t>(TW, jb.srcs.size() - i);
             tiles.push_back(t);
             for (int k = 0; k < t.ns; ++k) {
                 srcs.push_back(jb.srcs[i + k]);
@@ -1369,10 +1380,12 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     // two small-kernel workgroups or one workgroup-kernel workgroup per CU,
     // never more than the tiles, nor more slabs than fit in half the free
     // capacity
-    const size_t slab = (size_t)(wg ? nmax : SMALL_N) * 64;
-    // list entries: a vertex joins at most one level per lane, so 64 n bounds a tile
-    const size_t qcap = wg ? (size_t)nmax * 64 + 64 : slab;
-    const size_t per_wg = 16 * slab + 12 * qcap + (wg ? 4 * qcap + 32 * (size_t)nmax + 4 * ((size_t)nmax + 2) : 0);
+    const size_t slab = (size_t)(wg ? nmax : SMALL_N) * TW;
+    // list entries: a vertex joins at most one level per source, so 64 W n
+    // bounds a tile
+    const size_t qcap = wg ? (size_t)nmax * TW + TW : slab;
+    const size_t per_wg = 16 * slab + (4 + 8 * (size_t)W) * qcap +
+                          (wg ? 4 * qcap + 24 * (size_t)W * nmax + 12 * (size_t)nmax + 4 * ((size_t)nmax + 2) : 0);
     int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), (wg ? 1 : 2) * (int64_t)ctx->n_cu));
     if (wg) {
         const size_t room = ctx->mem_cap > ctx->mem_used ? (ctx->mem_cap - ctx->mem_used) / 2 : 0;
@@ -1381,9 +1394,10 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     if ((rc = ss.sig.ensure(ctx, 8 * slab * grid, "persistent sigma")) ||
         (rc = ss.tt.ensure(ctx, 8 * slab * grid, "persistent dependency terms")) ||
         (rc = ss.Q.ensure(ctx, 4 * qcap * grid, "persistent lists")) ||
-        (rc = ss.Qm.ensure(ctx, 8 * qcap * grid, "persistent lists")))
+        (rc = ss.Qm.ensure(ctx, 8 * (size_t)W * qcap * grid, "persistent lists")))
         return rc;
-    if (wg && ((rc = ss.masks.ensure(ctx, 24 * (size_t)nmax * grid, "workgroup masks")) ||
+    if (wg && W > 1 && (rc = ss.tag.ensure(ctx, 4 * (size_t)nmax * grid, "workgroup level tags"))) return rc;
+    if (wg && ((rc = ss.masks.ensure(ctx, 24 * (size_t)W * nmax * grid, "workgroup masks")) ||
                (rc = ss.qr.ensure(ctx, 4 * qcap * grid, "workgroup record offsets")) ||
                (rc = ss.ro.ensure(ctx, 8 * (size_t)nmax * grid, "workgroup record offsets")) ||
                (rc = ss.qoffs.ensure(ctx, 4 * ((size_t)nmax + 2) * grid, "workgroup level offsets"))))
@@ -1413,6 +1427,8 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     a.qcap = (int64_t)qcap;
     a.qr = wg ? (int32_t *)ss.qr.p : nullptr;
     a.ro = wg ? (int32_t *)ss.ro.p : nullptr;
+    a.spl = W;
+    a.tag = wg && W > 1 ? (int32_t *)ss.tag.p : nullptr;
     a.psig = probe && wg ? (uint64_t *)ss.psig.p : nullptr;
     a.lvl = probe ? (int32_t *)ss.lvl.p : nullptr;
     a.dbg = probe ? (double *)ss.dbg.p : nullptr;
@@ -1446,11 +1462,12 @@ static int persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg) {
     ss.pending = false;
     const hipStream_t sp = ss.run_stream;
     int rc;
-    int32_t h[2] = {0, 0};
+    int32_t h[3] = {0, 0, 0};
     if (ss.run.n_tiles > 0 &&
         ((rc = ctx_hip(ctx, hipMemcpyAsync(h, ss.run.counter, sizeof(h), hipMemcpyDeviceToHost, sp), "counter")) ||
          (rc = ctx_hip(ctx, hipStreamSynchronize(sp), "persistent kernel"))))
         return rc;
+    if (h[2]) return ctx_fail(ctx, BCX_E_OVERFLOW, "workgroup kernel: a tile's level lists exceeded their bound");
     if (h[1] > 0 || ss.run_narrow < ss.run_tiles.size()) {
         // tiles whose uint64 sigma overflowed added nothing: rerun them in
         // FP64, with the tiles that start there
@@ -1474,7 +1491,10 @@ static int persistent_finish(bcx_ctx *ctx, bcx_stats *st, bool wg) {
         if ((rc = persistent_pass(ctx, ss, wg, true, false))) return rc;
         ctx->timer.end(ctx);
         if (st) st->n_launches += 2;
-        if ((rc = ctx_hip(ctx, hipStreamSynchronize(sp), "persistent kernel"))) return rc;
+        if ((rc = ctx_hip(ctx, hipMemcpyAsync(h, ss.run.counter, sizeof(h), hipMemcpyDeviceToHost, sp), "counter")) ||
+            (rc = ctx_hip(ctx, hipStreamSynchronize(sp), "persistent kernel")))
+            return rc;
+        if (h[2]) return ctx_fail(ctx, BCX_E_OVERFLOW, "workgroup kernel: a tile's level lists exceeded their bound");
     }
     return BCX_OK;
 }
@@ -1497,7 +1517,7 @@ static void release_state(bcx_ctx *ctx, SmallState *&state) {
     SmallState &ss = *state;
     if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
     for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm,
-                    &ss.masks, &ss.qoffs, &ss.qr, &ss.ro, &ss.lvl, &ss.dbg, &ss.psig})
+                    &ss.masks, &ss.qoffs, &ss.qr, &ss.ro, &ss.tag, &ss.lvl, &ss.dbg, &ss.psig})
         b->release(ctx);
     delete state;
     state = nullptr;
@@ -1600,7 +1620,7 @@ int batched_schedule(bcx_ctx *ctx, DComp &dc, const bcx_opts *o, bcx_stats *st) 
     if (!rc) rc = time_multi(samp, base, &r_multi);
     if (!rc && need_kind) {
         double r_wg = 0.0;
-        rc = time_wg(sample_tiles(dc, std::max(1, ctx->n_cu), BATCH_W), &r_wg);
+        rc = time_wg(sample_tiles(dc, std::max(1, ctx->n_cu), (int64_t)BATCH_W * wg_spl(o)), &r_wg);
         kind = r_wg >= r_multi ? BCX_KERNEL_WG : BCX_KERNEL_MULTI;
     }
     MultiSched best = base;
@@ -1655,7 +1675,7 @@ int batched_debug_tile(bcx_ctx *ctx, const bcx_component *comp, const int32_t *s
     const int kind = batched_kind(dc, o->flags);
     // sources per lane of the tile: the multi-launch tiles' schedule (the
     // other kernels run 64-source tiles)
-    const int spl = kind == BCX_KERNEL_MULTI ? multi_sched(dc, o).spl : 1;
+    const int spl = kind == BCX_KERNEL_MULTI ? multi_sched(dc, o).spl : (kind == BCX_KERNEL_WG ? wg_spl(o) : 1);
     if (ns > BATCH_W * spl) {
         bcx_dgraph_free(ctx, g);
         return ctx_fail(ctx, BCX_E_ARG, "debug tile: more sources than the kernel's tile holds");

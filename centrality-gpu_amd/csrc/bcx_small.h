@@ -19,7 +19,7 @@ struct SmallComp {
 struct SmallTile {
     int32_t comp;  // index into comps
     int32_t src0;  // first source in srcs / ws
-    int32_t ns;    // sources in the tile (<= 64)
+    int32_t ns;    // sources in the tile (<= 64; <= 64 W in the workgroup kernel)
     int32_t pad;
 };
 
@@ -29,7 +29,7 @@ struct SmallArgs {
     int32_t n_tiles;
     const int32_t *srcs;
     const double *ws;
-    int32_t *counter;   // [0] work counter, [1] tiles listed for an FP64 rerun
+    int32_t *counter;   // [0] work counter, [1] tiles listed for an FP64 rerun, [2] a full list
     int32_t *redo;      // [n_tiles] those tiles
     uint64_t *sig;      // per-workgroup slabs of slab words
     void *tt;
@@ -47,6 +47,11 @@ struct SmallArgs {
     // word per lane of each list entry) located through these offsets
     int32_t *qr;        // [grid][qcap] record offset of each list entry
     int32_t *ro;        // [grid][2 * nmax] record offset of each vertex's current entry
+    // workgroup-per-tile kernel: sources per lane W (0 or 1: 64-source tiles;
+    // 2 or 4: tiles of 64 W sources, source k + 64 h on lane k, masks of W
+    // words) and, for W > 1, the level each vertex was last listed at
+    int32_t spl;
+    int32_t *tag;       // [grid][nmax]
     uint64_t *psig;     // probe only: [grid][slab] path counts per (vertex, lane)
     // probe only (null in runs): per-workgroup slabs like sig, holding the
     // level and the dependency of every (vertex, lane) of the last tile
@@ -62,7 +67,7 @@ hipError_t launch_small_reset(int32_t *counter, hipStream_t st);
 
 // Components up to WG_N vertices whose levels are narrow (deep BFS) run one
 // workgroup per tile through every level, like the small kernel, with the
-// per-vertex state in HBM.
+// per-vertex state in HBM (a.spl sources per lane).
 constexpr int WG_N = 1 << 17;
 hipError_t launch_wg(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st);
 

@@ -157,9 +157,10 @@ typedef struct {
     int32_t        flags;         /* BCX_FLAG_* bits                          */
     int32_t        blocks_per_cu; /* batched multi-launch tiles: level-kernel
                                      blocks per compute unit (0 = tuned)     */
-    int32_t        sources_per_lane; /* batched multi-launch tiles: 1, 2 or 4
-                                     sources per lane of a wave64, tiles of
-                                     64, 128 or 256 sources (0 = tuned)      */
+    int32_t        sources_per_lane; /* batched multi-launch and workgroup
+                                     tiles: 1, 2 or 4 sources per lane of a
+                                     wave64, tiles of 64, 128 or 256 sources
+                                     (0 = tuned / the workgroup default)     */
 } bcx_opts;
 
 /* Batched path: run components of up to 2048 vertices through the

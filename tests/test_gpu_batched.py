@@ -290,12 +290,14 @@ def grid_edges(w, h, base=0):
     return e
 
 
+@pytest.mark.parametrize("spl", [0, 2, 4])
 @pytest.mark.parametrize("prec,tol", [(bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)])
-def test_wg_kernel_deep_components(engine, prec, tol):
+def test_wg_kernel_deep_components(engine, prec, tol, spl):
     """Two grids (60x60 and 45x50, sigma past 2^64 in both) joined by a
     bridge: both biconnected components run in one workgroup-per-tile launch
     (uint64 pass, then the FP64 rerun of the overflowed tiles) and match the
-    oracle and the multi-launch tiles."""
+    oracle and the multi-launch tiles; with tiles of 64, 128 and 256
+    sources."""
     e = grid_edges(60, 60) + grid_edges(45, 50, base=3600) + [(3599, 3600)]
     n = 3600 + 45 * 50
     h = bcx.Host.from_edges(n, e).decompose()
@@ -303,30 +305,31 @@ def test_wg_kernel_deep_components(engine, prec, tol):
     assert sorted(c.n for c in comps) == [2250, 3600]
     rp, col = oracle.csr_from_edges(n, e)
     want = oracle.brandes_bc(rp, col, threads=16)
-    got, st = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_WG_KERNEL)
+    got, st = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_WG_KERNEL, sources_per_lane=spl)
     assert st["n_f64_sigma"] > 0
     mx, mean, zeros = relerr(got, want)
-    print("wg prec", prec, "maxrel", mx, "launches", st["n_launches"], flush=True)
+    print("wg prec", prec, "spl", spl, "maxrel", mx, "launches", st["n_launches"], flush=True)
     assert mx < tol and zeros
     ref, _ = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_NO_WG_KERNEL)
     mx2, _, _ = relerr(got, ref)
     assert mx2 < 2 * tol
 
 
+@pytest.mark.parametrize("spl", [0, 2, 4])
 @pytest.mark.parametrize("cfg,scale", [(2, 0.003), (4, 0.1), (5, 0.004)])
-def test_wg_kernel_generated(engine, cfg, scale):
+def test_wg_kernel_generated(engine, cfg, scale, spl):
     """Forced onto the workgroup kernel, generated graphs (a power-law core
-    with hubs, the chained grid, R-MAT) match the oracle; the grid is also
-    picked by the narrow-level rule without the flag."""
+    with hubs, the chained grid, R-MAT) match the oracle, with tiles of 64,
+    128 and 256 sources."""
     n, uv = bcx.generate(cfg, 0, scale)
     h = bcx.Host.from_edges(n, uv).decompose()
     assert max(c.n for c in h.components()) > 2048
     rp, col = oracle.csr_from_edges(n, uv)
     want = oracle.brandes_bc(rp, col, threads=16)
     for prec, tol in ((bcx.PREC_FP64, 1e-9), (bcx.PREC_FP32C, 1e-6)):
-        got, st = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_WG_KERNEL)
+        got, st = engine.host_bc(h, path=B, precision=prec, flags=bcx.FLAG_WG_KERNEL, sources_per_lane=spl)
         mx, mean, zeros = relerr(got, want)
-        print("cfg", cfg, "prec", prec, "maxrel", mx, "launches", st["n_launches"], flush=True)
+        print("cfg", cfg, "spl", spl, "prec", prec, "maxrel", mx, "launches", st["n_launches"], flush=True)
         assert mx < tol and zeros
 
 

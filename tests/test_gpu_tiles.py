@@ -25,9 +25,10 @@ SMALL = 0
 WG = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_WG_KERNEL
 MULTI = bcx.FLAG_NO_SMALL_KERNEL | bcx.FLAG_NO_WG_KERNEL
 KERNELS = {"small": (SMALL, bcx.KERNEL_SMALL), "wg": (WG, bcx.KERNEL_WG), "multi": (MULTI, bcx.KERNEL_MULTI),
-           "multi64": (MULTI, bcx.KERNEL_MULTI), "multi128": (MULTI, bcx.KERNEL_MULTI)}
-# sources per lane of the multi-launch tiles ("multi": the default, 4)
-SPL = {"multi64": 1, "multi128": 2}
+           "multi64": (MULTI, bcx.KERNEL_MULTI), "multi128": (MULTI, bcx.KERNEL_MULTI),
+           "wg128": (WG, bcx.KERNEL_WG), "wg256": (WG, bcx.KERNEL_WG)}
+# sources per lane of the multi-launch and workgroup tiles ("multi", "wg": the defaults)
+SPL = {"multi64": 1, "multi128": 2, "wg128": 2, "wg256": 4}
 
 
 def comps_of(n, uv, chains=True):
@@ -53,10 +54,11 @@ def test_golden_tiles(engine, goldens, prec, kernel):
 
 @pytest.mark.parametrize("spl", [2, 4])
 @pytest.mark.parametrize("cfg,scale", [(1, 0.1), (2, 0.002), (4, 0.02), (5, 0.002)])
-def test_generated_wide_tiles(engine, cfg, scale, spl):
-    """Whole tiles of 128 or 256 sources on the multi-launch kernels (two or
-    four sources per lane): every source against the oracle."""
-    flags, kid = KERNELS["multi"]
+@pytest.mark.parametrize("kernel", ["multi", "wg"])
+def test_generated_wide_tiles(engine, cfg, scale, spl, kernel):
+    """Whole tiles of 128 or 256 sources on the multi-launch and workgroup
+    kernels (two or four sources per lane): every source against the oracle."""
+    flags, kid = KERNELS[kernel]
     tw = 64 * spl
     n, uv = bcx.generate(cfg, 0, scale)
     comps = comps_of(n, uv)
@@ -73,7 +75,7 @@ def test_generated_wide_tiles(engine, cfg, scale, spl):
     check_tile(engine, c, order[:tw], bcx.PREC_FP64, flags, kid, tag=f"cfg{cfg}/w0/fp64", spl=spl)
 
 
-@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64", "multi128"])
+@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64", "multi128", "wg128", "wg256"])
 @pytest.mark.parametrize("cfg,scale", [(1, 0.1), (2, 0.002), (4, 0.02), (5, 0.002)])
 def test_generated_tiles(engine, cfg, scale, kernel):
     """Whole tuned tiles (bcx_batched_order) of the largest component of each
@@ -96,7 +98,7 @@ def test_generated_tiles(engine, cfg, scale, kernel):
         check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, tag=f"cfg{cfg}/t{t}/{kernel}", spl=SPL.get(kernel, 0))
 
 
-@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64", "multi128"])
+@pytest.mark.parametrize("kernel", ["small", "wg", "multi", "multi64", "multi128", "wg128", "wg256"])
 def test_fp64_counts_grid_tile(engine, kernel):
     """40x40 grid: corner path counts C(78,39) > 2^64, so a tile holding a
     corner runs with FP64 counts on every kernel, bit-exact in pull order."""
@@ -107,9 +109,12 @@ def test_fp64_counts_grid_tile(engine, kernel):
     spl = SPL.get(kernel, 0)
     check_tile(engine, c, srcs, bcx.PREC_FP64, flags, kid, want_f64=True, tag=f"grid40/{kernel}", spl=spl)
     check_tile(engine, c, srcs, bcx.PREC_FP32C, flags, kid, want_f64=True, tag=f"grid40/{kernel}/fp32c", spl=spl)
-    if kernel == "multi128":
+    if kernel in ("multi128", "wg128"):
         wide = srcs + [v for v in range(400, 464)]
-        check_tile(engine, c, wide, bcx.PREC_FP32C, flags, kid, want_f64=True, tag="grid40/multi/128", spl=2)
+        check_tile(engine, c, wide, bcx.PREC_FP32C, flags, kid, want_f64=True, tag=f"grid40/{kernel}/128", spl=2)
+    if kernel == "wg256":
+        wide = srcs + [v for v in range(400, 592)]
+        check_tile(engine, c, wide, bcx.PREC_FP64, flags, kid, want_f64=True, tag="grid40/wg/256", spl=4)
     if kernel == "multi":
         wide = srcs + [v for v in range(400, 592)]
         check_tile(engine, c, wide, bcx.PREC_FP32C, flags, kid, want_f64=True, tag="grid40/multi/256")
