@@ -725,33 +725,6 @@ __device__ __forceinline__ double seg_term(int a, int bl, double sA, double sB, 
     return term;
 }
 
-// The chain terms of endpoint e (vertex at level l, count sv) for source
-// slot i = h * 64 + lane: every attached chain is a segment from e to its
-// other endpoint, except the chain holding the source x_j itself, which is the
-// segment from e to the source (level 0, count 1, j arcs from L or K - j from
-// R).
-template <int SM, int W>
-__device__ double chain_rho(const BatchDev &b, int e, int l, double sv, int lane, int h) {
-    const int i = h * 64 + lane;
-    const int myc = b.lane_ch[i], myj = b.lane_ch[64 * W + i];
-    double rho = 0.0;
-    for (int q = b.ep_off[e]; q < b.ep_off[e + 1]; ++q) {
-        const int code = b.ep_list[q];
-        const int c = code >> 1, side = code & 1;
-        const int off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
-        const ChainPfx P{b.ch_pre + off, side ? K - 1 : 0, side ? -1 : 1};
-        if (c == myc) {
-            rho += seg_term(l, 0, sv, 1.0, side ? K - myj : myj, P);
-        } else {
-            const int o = side ? b.ch_L[c] : b.ch_R[c];
-            const int lv = b.ep_lvl[((int64_t)b.ep_idx[o] * 64 + lane) * W + h];
-            const double so = sg_val<SM>(sg<SM>(b, ((int64_t)o * 64 + lane) * W + h));
-            rho += seg_term(l, lv, sv, so, K, P);
-        }
-    }
-    return rho;
-}
-
 // rho_v = sum over successors w (sources in succ[w] & M) of t_w.
 template <typename TT, int W>
 __device__ __forceinline__ void rho_sum(const BatchDev &b, const uint64_t *succ, const Msk<W> &M, int a0, int a1,
@@ -1115,15 +1088,52 @@ __global__ __launch_bounds__(BB) void k_b_chain_terms(BatchDev b) {
     const int lane = threadIdx.x & 63;
     const int ns = b.ctr[5];
     for (int e = gwave(); e < b.n_ep; e += nwaves()) {
-        const int code = b.ep_list[b.ep_off[e]];  // (an endpoint has a chain: its vertex is that chain's end)
-        const int v = (code & 1) ? b.ch_R[code >> 1] : b.ch_L[code >> 1];
-#pragma unroll 1
+        const int q0 = b.ep_off[e], q1 = b.ep_off[e + 1];
+        const int code0 = b.ep_list[q0];  // (an endpoint has a chain: its vertex is that chain's end)
+        const int v = (code0 & 1) ? b.ch_R[code0 >> 1] : b.ch_L[code0 >> 1];
+        // per source slot k + 64 h: the endpoint's level and count, the
+        // source's own chain and position (-1: a tile vertex)
+        int lv[W], myc[W], myj[W];
+        double sv[W], rho[W];
+#pragma unroll
         for (int h = 0; h < W; ++h) {
-            if (h * 64 + lane >= ns) continue;
-            const int64_t c = ((int64_t)e * 64 + lane) * W + h;
-            const double sv = sg_val<SM>(sg<SM>(b, ((int64_t)v * 64 + lane) * W + h));
-            b.ep_rho[c] = chain_rho<SM, W>(b, e, b.ep_lvl[c], sv, lane, h);
+            const int i = h * 64 + lane;
+            const bool act = i < ns;
+            lv[h] = act ? b.ep_lvl[((int64_t)e * 64 + lane) * W + h] : 0;
+            sv[h] = act ? sg_val<SM>(sg<SM>(b, ((int64_t)v * 64 + lane) * W + h)) : 1.0;
+            myc[h] = act ? b.lane_ch[i] : -1;
+            myj[h] = act ? b.lane_ch[64 * W + i] : 0;
+            rho[h] = 0.0;
         }
+        // every attached chain is a segment from the endpoint to its other
+        // end, except the chain holding the source x_j itself, which is the
+        // segment from the endpoint to the source (level 0, count 1, j arcs
+        // from L or K - j from R); the other end's level and count are loaded
+        // for all W sources of a lane at once
+        for (int q = q0; q < q1; ++q) {
+            const int code = b.ep_list[q];
+            const int c = code >> 1, side = code & 1;
+            const int off = b.ch_off[c], K = b.ch_off[c + 1] - off + 1;
+            const ChainPfx P{b.ch_pre + off, side ? K - 1 : 0, side ? -1 : 1};
+            const int o = side ? b.ch_L[c] : b.ch_R[c];
+            const int64_t eo = (int64_t)b.ep_idx[o] * 64 + lane;
+            int lo[W];
+            double so[W];
+#pragma unroll
+            for (int h = 0; h < W; ++h) {
+                lo[h] = b.ep_lvl[eo * W + h];
+                so[h] = sg_val<SM>(sg<SM>(b, ((int64_t)o * 64 + lane) * W + h));
+            }
+#pragma unroll
+            for (int h = 0; h < W; ++h) {
+                if (h * 64 + lane >= ns) continue;
+                rho[h] += c == myc[h] ? seg_term(lv[h], 0, sv[h], 1.0, side ? K - myj[h] : myj[h], P)
+                                      : seg_term(lv[h], lo[h], sv[h], so[h], K, P);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < W; ++h)
+            if (h * 64 + lane < ns) b.ep_rho[((int64_t)e * 64 + lane) * W + h] = rho[h];
     }
 }
 
