@@ -4,10 +4,11 @@ plain per-source path, on the 1M-vertex mixed graph (Chung-Lu power-law core
 
 A step is one pass of the hot path over one slice of the job's sources: the
 job's sources (every vertex of every biconnected component with more than two
-vertices) are ordered by BFS rank inside their component and cut into tiles of
-64 consecutive sources, and slice i holds every S-th tile (i, i+S, ...), so
-each slice is a uniform sample of the job with the tile structure of a full
-run.  A step is credited n*m * (slice sources / job sources) traversals
+vertices) are taken in the library's tile order inside their component and
+cut into blocks of 256 consecutive sources (a whole number of the library's
+tiles of 64, 128 or 256 sources), and slice i holds every S-th block (i, i+S,
+...), so each slice is a uniform sample of the job with the tile structure of
+a full run.  A step is credited n*m * (slice sources / job sources) traversals
 (MTEPS = n*m / t, n and m of the input graph).  Inputs are resident in HBM
 before the timed region; each step is one bcx_run through the C-ABI.
 
@@ -228,7 +229,7 @@ def main():
     # tile-order tuner runs here, once per component, outside the timing)
     t0 = time.perf_counter()
     orders = [g.batched_order(ci) for ci in range(len(comps))]
-    slices = make_slices(comps, offsets, orders, args.slices)
+    slices = make_slices(comps, offsets, orders, args.slices, tile=256)
     n_tiles = sum((c.n + 63) // 64 for c in comps)
     plain_slices = max(1, n_tiles // max(1, args.plain_sources // 64))
     pslices = make_slices(comps, offsets, orders, plain_slices)
@@ -344,7 +345,7 @@ def main():
                      if traffic and tk_b > 0 else None,
                      "launches_per_step": round(launches, 1),
                      "algorithmic_bytes_per_step": round(balg_b / K),
-                     "kernel": "batched kernels of a step (k_b_expand, k_b_pull, k_b_back, k_b_chains; "
+                     "kernel": "batched kernels of a step (k_b_expand, k_b_pull, k_b_back, the chain kernels; "
                                "k_small; k_wg)"},
         "cpu_baseline": cpu,
     }
