@@ -1,10 +1,12 @@
-// Host driver of the batched multi-source path: tiles of up to 64 sources
-// of one component, forward level lists built on the device, FP32-compensated
-// (or FP64) rho-form backward.  Path counts start in 32-bit words (half the
-// bytes of every sigma gather); a tile whose counts leave the word's range
-// is redone with 64-bit words, and past those with FP64.  Sources are ordered by a BFS from the component's
-// highest-degree vertex so the 64 sources of a tile are close together and
-// their frontiers overlap (fewer (vertex, level) list entries per source).
+// Host driver of the batched multi-source path: tiles of up to 64 W sources
+// (W = 1, 2 or 4 per lane of a wave64) of one component, forward level lists
+// built on the device, FP32-compensated (or FP64) rho-form backward.  Path
+// counts start in 32-bit words (half the bytes of every sigma gather); a tile
+// whose counts leave the word's range is redone with 64-bit words, and past
+// those with FP64.  Sources are ordered (BFS rank, landmark distances or graph
+// balls, whichever sample tiles show to need less work) so the sources of a
+// tile are close together and their frontiers overlap (fewer (vertex, level)
+// list entries per source).
 //
 // Several tiles run at once, one per slot, each slot with its own stream and
 // state: the many small levels of one tile (near its sources, along long
@@ -33,7 +35,7 @@ using namespace bcx;
 // T = 1024 (16.9k slices) beats 512 (28.9k) by 12% and 256 (86.6k) by 85%.
 constexpr int HUB_MIN = 512, HUB_SLICES_PER_CU = 80, HUB_LEVELS = 1024;
 // Tiles in flight on their own streams: 8 measured 4-10% above 4 on the
-// Chung-Lu and R-MAT cores (6 equals 4, 12 and 16 equal 8).
+// Chung-Lu and R-MAT cores (6 equals 4, 12 and 16 within 1.2% of 8).
 constexpr int MAX_SLOTS = 8;
 
 // A tile's level launches, captured once and replayed for every tile whose
