@@ -824,7 +824,7 @@ __global__ __launch_bounds__(BB) void k_b_back(BatchDev b, int l) {
     // as one flat range (lane per arc) in windows of WIN: successor masks
     // are gathered for 64 arcs at once and the (entry, successor, sources)
     // triples that matter are compacted in LDS; the t rows of those triples
-    // are then gathered UNR at a time, accumulated per entry in arc order.
+    // are then gathered UW at a time, accumulated per entry in arc order.
     for (int base = q0 + gwave() * EB; base < q1; base += nwaves() * EB) {
         const int e = base + lane;
         int v0 = 0, r0 = 0, r1 = 0, ep0 = -1;
