@@ -1327,8 +1327,14 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     SmallState *&state = wg ? ctx->wgk : ctx->small;
     if (!state) state = new SmallState();
     SmallState &ss = *state;
-    int64_t nmax = 1;
-    for (const SmallJob &jb : jobs) nmax = std::max<int64_t>(nmax, jb.dc->n);
+    int64_t nmax = 1, csr = 0;
+    for (const SmallJob &jb : jobs) {
+        nmax = std::max<int64_t>(nmax, jb.dc->n);
+        csr = std::max<int64_t>(csr, jb.dc->n + 1 + jb.dc->arcs);
+    }
+    // small kernel: a tile's CSR joins the LDS state when it fits beside the
+    // masks
+    csr = wg ? 0 : small_csr_words(nmax, csr);
     std::vector<SmallComp> comps;
     std::vector<SmallTile> tiles;
     std::vector<int32_t> srcs;
@@ -1379,16 +1385,17 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
         (rc = ss.counter.ensure(ctx, 64, "small counters")) ||
         (rc = ss.redo.ensure(ctx, 4 * tiles.size(), "small redo list")))
         return rc;
-    // two small-kernel workgroups or one workgroup-kernel workgroup per CU,
-    // never more than the tiles, nor more slabs than fit in half the free
-    // capacity
-    const size_t slab = (size_t)(wg ? nmax : SMALL_N) * TW;
+    // as many small-kernel workgroups per CU as stay resident (LDS,
+    // registers) or one workgroup-kernel workgroup per CU, never more than
+    // the tiles, nor more slabs than fit in half the free capacity
+    const size_t slab = (size_t)nmax * TW;
     // list entries: a vertex joins at most one level per source, so 64 W n
     // bounds a tile
     const size_t qcap = wg ? (size_t)nmax * TW + TW : slab;
     const size_t per_wg = 16 * slab + (4 + 8 * (size_t)W) * qcap +
                           (wg ? 4 * qcap + 24 * (size_t)W * nmax + 12 * (size_t)nmax + 4 * ((size_t)nmax + 2) : 0);
-    int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), (wg ? 1 : 2) * (int64_t)ctx->n_cu));
+    const int64_t per_cu = wg ? 1 : small_blocks_per_cu(nmax, csr);
+    int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), per_cu * (int64_t)ctx->n_cu));
     if (wg) {
         const size_t room = ctx->mem_cap > ctx->mem_used ? (ctx->mem_cap - ctx->mem_used) / 2 : 0;
         grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)(room / per_wg)));
@@ -1426,6 +1433,7 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
     a.masks = wg ? (uint64_t *)ss.masks.p : nullptr;
     a.qoffs = wg ? (int32_t *)ss.qoffs.p : nullptr;
     a.nmax = nmax;
+    a.csr_cap = csr;
     a.qcap = (int64_t)qcap;
     a.qr = wg ? (int32_t *)ss.qr.p : nullptr;
     a.ro = wg ? (int32_t *)ss.ro.p : nullptr;

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(64) void k_na_remap(unsigned char *b) {
     if (threadIdx.x == 0) ctr[N2] = k;
 }
 
-static SmallArgs pass_args(unsigned char *b, bool second) {
+static SmallArgs pass_args(unsigned char *b, bool second, int32_t n) {
     SmallArgs a{};
     a.comps = (const SmallComp *)(b + O_COMP);
     a.tiles = (const SmallTile *)(b + (second ? O_TILES2 : O_TILES1));
@@ -154,6 +154,7 @@ static SmallArgs pass_args(unsigned char *b, bool second) {
     a.Qm = (uint64_t *)(b + O_QM);
     a.slab = (int64_t)NA_SLAB;
     a.limit = 0;
+    a.nmax = n;  // sizes the kernel's LDS
     return a;
 }
 
@@ -195,10 +196,10 @@ extern "C" int bcx_bc_small_noalloc(bcx_ctx *ctx, int32_t n, int64_t arcs, const
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(k_na_setup, dim3(1), dim3(256), 0, st, x);
     if ((rc = ctx_hip(ctx, hipGetLastError(), "no-alloc setup"))) return rc;
-    if ((rc = ctx_hip(ctx, launch_small(pass_args(base, false), false, fp32, grid, st), "no-alloc pass"))) return rc;
+    if ((rc = ctx_hip(ctx, launch_small(pass_args(base, false, n), false, fp32, grid, st), "no-alloc pass"))) return rc;
     hipLaunchKernelGGL(k_na_remap, dim3(1), dim3(64), 0, st, base);
     if ((rc = ctx_hip(ctx, hipGetLastError(), "no-alloc remap"))) return rc;
-    if ((rc = ctx_hip(ctx, launch_small(pass_args(base, true), true, fp32, grid, st), "no-alloc FP64 pass")))
+    if ((rc = ctx_hip(ctx, launch_small(pass_args(base, true, n), true, fp32, grid, st), "no-alloc FP64 pass")))
         return rc;
     int32_t status[16];
     if ((rc = ctx_hip(ctx, hipMemcpyAsync(status, base + O_CTR, sizeof(status), hipMemcpyDeviceToHost, st),

@@ -488,7 +488,7 @@ int bcx_upload(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps, bcx_dg
     int64_t off = 0;
     for (int32_t i = 0; i < n_comps; ++i) {
         const bcx_component &c = comps[i];
-        if (c.n <= 0 || !c.row_ptr || !c.col || !c.reach || c.arcs != (int64_t)c.row_ptr[c.n] ||
+        if (c.n <= 0 || !c.row_ptr || !c.col || !c.reach || c.row_ptr[0] != 0 || c.arcs != (int64_t)c.row_ptr[c.n] ||
             c.arcs >= INT32_MAX || (c.n_chains > 0 && (!c.chain_ptr || !c.chain_nodes || !c.chain_ends))) {
             bcx_dgraph_free(ctx, g);
             return ctx_fail(ctx, BCX_E_ARG, "component " + std::to_string(i) + " is malformed");

@@ -8,6 +8,13 @@
 namespace bcx {
 
 constexpr int SMALL_N = 2048;  // largest component handled by one workgroup
+// LDS of a small-kernel workgroup: 24 B + 4 B per vertex of the largest
+// component, plus room for a tile's CSR, at most SMALL_LDS bytes (more
+// keeps the multi-launch kernels' blocks off the CU while it runs: 2%
+// slower on the 12.8M-vertex workload at 160 KiB)
+constexpr size_t SMALL_LDS = 65536;
+size_t small_lds_bytes(int64_t cap, int64_t csr);
+int64_t small_csr_words(int64_t nmax, int64_t csr);
 
 struct SmallComp {
     int32_t n;
@@ -43,6 +50,10 @@ struct SmallArgs {
     int32_t *qoffs;     // [grid][nmax + 2]
     int64_t nmax;       // largest component
     int64_t qcap;       // list entries per workgroup (64 * nmax + 64)
+    // small kernel: LDS words for a copy of a tile's CSR (n + 1 row
+    // pointers and the arcs); tiles whose CSR is larger, and every tile when
+    // 0, read the CSR from HBM
+    int64_t csr_cap;
     // workgroup-per-tile kernel: sig / tt hold lane-compacted records (one
     // word per lane of each list entry) located through these offsets
     int32_t *qr;        // [grid][qcap] record offset of each list entry
@@ -63,6 +74,7 @@ struct SmallArgs {
 };
 
 hipError_t launch_small(const SmallArgs &a, bool f64, bool fp32, int grid, hipStream_t st);
+int small_blocks_per_cu(int64_t nmax, int64_t csr);
 hipError_t launch_small_reset(int32_t *counter, hipStream_t st);
 
 // Components up to WG_N vertices whose levels are narrow (deep BFS) run one
