@@ -71,12 +71,13 @@ struct TileCtx {
     double *pdl;
 };
 
-// One tile (component, <= 64 sources) through every level.  CSRL: the
-// tile's CSR was copied to LDS at rp / col (the loops over neighbours then
-// read LDS instead of L2).  Returns false when the uint64 sigma overflowed.
-template <bool F64, typename TT, int SB, bool CSRL>
+// One tile (component, <= 64 sources) through every level, with the CSR at
+// rp / col: a copy in LDS (PI = int32_t; the loops over neighbours then read
+// LDS instead of L2) or the component's in HBM (PI = gint).  Returns false
+// when the uint64 sigma overflowed.
+template <bool F64, typename TT, int SB, typename PI>
 __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &tl, const SmallComp &cp,
-                                           const int32_t *rp, const int32_t *col, const TileCtx<TT> &x) {
+                                           const PI *rp, const PI *col, const TileCtx<TT> &x) {
     constexpr int SW = SB / 64;  // waves per workgroup
     using Acc = typename std::conditional<sizeof(TT) == 4, KahanS, SumS>::type;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -248,7 +249,7 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
             if (lane == 0) own[v] = M;
             // the vertex's own words load while its successors are summed
             const uint64_t sx = (M & me) ? sig[(int64_t)v * 64 + lane] : 0;
-            const int rv = cp.reach[v];
+            const int rv = glb(cp.reach)[v];
             const int a0 = rp[v], a1 = rp[v + 1];
             Acc acc;
             if (lb < D) {
@@ -294,7 +295,7 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
                 if (x.pdl) x.pdl[(int64_t)v * 64 + lane] = delta;
             }
             contrib = wave_sum(contrib);
-            if (lane == 0 && contrib != 0.0) unsafeAtomicAdd(&cp.bc[v], contrib);
+            if (lane == 0 && contrib != 0.0) gadd(glb(cp.bc) + v, contrib);
         }
         __syncthreads();
         // level lb+1's masks (the only non-zero words of succ) are done
@@ -340,16 +341,17 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
         const SmallTile tl = a.tiles[item];
         const SmallComp cp = a.comps[tl.comp];
         const int n = cp.n;
-        const int r0 = cp.rp[0], m = cp.rp[n] - r0;
+        const gint *grp = glb(cp.rp), *gcol = glb(cp.col);
+        const int r0 = grp[0], m = grp[n] - r0;
         bool done;
         if (a.csr_cap > 0 && (int64_t)n + 1 + m <= a.csr_cap) {
             int *lcol = lrp + n + 1;
-            for (int v = tid; v <= n; v += SB) lrp[v] = cp.rp[v] - r0;
-            for (int j = tid; j < m; j += SB) lcol[j] = cp.col[r0 + j];
+            for (int v = tid; v <= n; v += SB) lrp[v] = grp[v] - r0;
+            for (int j = tid; j < m; j += SB) lcol[j] = gcol[r0 + j];
             // (small_tile's first barrier orders these stores before use)
-            done = small_tile<F64, TT, SB, true>(a, tl, cp, lrp, lcol, x);
+            done = small_tile<F64, TT, SB, int32_t>(a, tl, cp, lrp, lcol, x);
         } else {
-            done = small_tile<F64, TT, SB, false>(a, tl, cp, cp.rp, cp.col, x);
+            done = small_tile<F64, TT, SB, gint>(a, tl, cp, grp, gcol, x);
         }
         if (!done && tid == 0) {
             const int k = atomicAdd(&a.counter[1], 1);
