@@ -120,11 +120,14 @@ struct bcx_ctx {
     bcx::Timer timer, timer_chain;
     bcx::DBuf counters, bc_acc, plain_scratch, rows_buf;
     bcx::DBuf w_src, w_aux, w_mode, w_slots;
+    bcx::DBuf bc_pack, bc_chunks;  // result download: packed slices and their chunk list
+    double *bc_pin = nullptr;      // pinned host staging of the result download
+    size_t bc_pin_bytes = 0;
     BatchedState *batched = nullptr;
     SmallState *small = nullptr;
     SmallState *wgk = nullptr;  // workgroup-per-tile kernel buffers
     std::vector<bcx::DBuf *> all_bufs() {
-        return {&counters, &bc_acc, &plain_scratch, &rows_buf, &w_src, &w_aux, &w_mode, &w_slots};
+        return {&counters, &bc_acc, &plain_scratch, &rows_buf, &w_src, &w_aux, &w_mode, &w_slots, &bc_pack, &bc_chunks};
     }
 };
 

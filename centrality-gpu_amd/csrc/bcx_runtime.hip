@@ -463,6 +463,7 @@ void bcx_destroy(bcx_ctx *ctx) {
     small_release(ctx);
     ctx->timer.destroy();
     ctx->timer_chain.destroy();
+    if (ctx->bc_pin) (void)hipHostFree(ctx->bc_pin);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -575,6 +576,72 @@ void bcx_dgraph_free(bcx_ctx *ctx, bcx_dgraph *g) {
         g->pool.release(ctx);
     }
     delete g;
+}
+
+// A slice of the BC accumulator that comes back to the host: bc[src, src +
+// len) lands at packed[dst, dst + len).
+struct BcChunk {
+    int64_t src, dst;
+    int32_t len, pad;
+};
+constexpr int32_t BC_CHUNK = 8192;
+
+__global__ void k_bc_gather(const double *bc, double *packed, const BcChunk *ch) {
+    const BcChunk c = ch[blockIdx.x];
+    for (int i = threadIdx.x; i < c.len; i += blockDim.x) packed[c.dst + i] = bc[c.src + i];
+}
+
+// Adds the run's BC to bc_out.  A run over a source subset leaves the
+// slices of components without sources at the zeros they were set to, so
+// only the other slices come back: gathered into one packed buffer on the
+// device, copied through pinned staging and added slice by slice.
+static int download_bc(bcx_ctx *ctx, bcx_dgraph *g, const double *bc, const std::vector<char> *touched,
+                       double *bc_out) {
+    std::vector<BcChunk> chunks;
+    int64_t packed = 0;
+    for (size_t ci = 0; ci < g->comps.size(); ++ci) {
+        if (touched && !(*touched)[ci]) continue;
+        const DComp &dc = g->comps[ci];
+        for (int64_t i = 0; i < dc.n; i += BC_CHUNK) {
+            BcChunk c{};
+            c.src = dc.offset + i;
+            c.dst = packed;
+            c.len = (int32_t)std::min<int64_t>(BC_CHUNK, dc.n - i);
+            chunks.push_back(c);
+            packed += c.len;
+        }
+    }
+    if (packed == 0) return BCX_OK;
+    const size_t bytes = sizeof(double) * (size_t)packed;
+    if (ctx->bc_pin_bytes < bytes) {
+        if (ctx->bc_pin) (void)hipHostFree(ctx->bc_pin);
+        ctx->bc_pin = nullptr;
+        ctx->bc_pin_bytes = 0;
+        int rc = ctx_hip(ctx, hipHostMalloc((void **)&ctx->bc_pin, bytes, hipHostMallocDefault), "result staging");
+        if (rc) return rc;
+        ctx->bc_pin_bytes = bytes;
+    }
+    const double *from = bc;
+    int rc;
+    if (packed < g->total_n) {
+        if ((rc = upload(ctx, chunks.data(), sizeof(BcChunk) * chunks.size(), ctx->bc_chunks, "result chunks")) ||
+            (rc = ctx->bc_pack.ensure(ctx, bytes, "packed result")))
+            return rc;
+        hipLaunchKernelGGL(k_bc_gather, dim3((unsigned)chunks.size()), dim3(256), 0, ctx->stream, bc,
+                           (double *)ctx->bc_pack.p, (const BcChunk *)ctx->bc_chunks.p);
+        if ((rc = ctx_hip(ctx, hipGetLastError(), "result gather"))) return rc;
+        from = (const double *)ctx->bc_pack.p;
+    }
+    if ((rc = ctx_hip(ctx, hipMemcpyAsync(ctx->bc_pin, from, bytes, hipMemcpyDeviceToHost, ctx->stream),
+                      "bc copy")) ||
+        (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "bc copy")))
+        return rc;
+    for (const BcChunk &c : chunks) {
+        double *o = bc_out + c.src;
+        const double *h = ctx->bc_pin + c.dst;
+        for (int32_t i = 0; i < c.len; ++i) o[i] += h[i];
+    }
+    return BCX_OK;
 }
 
 int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, bcx_stats *st) {
@@ -701,12 +768,12 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
     s->t_chain_s = ctx->timer_chain.collect();
     s->t_kernel_s = ctx->timer.collect() + s->t_chain_s;
     if (bc_out && g->total_n > 0) {
-        std::vector<double> h((size_t)g->total_n);
-        rc = ctx_hip(ctx, hipMemcpyAsync(h.data(), bc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream),
-                     "bc copy");
-        if (!rc) rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "bc copy");
-        if (rc) return rc;
-        for (size_t i = 0; i < h.size(); ++i) bc_out[i] += h[i];
+        std::vector<char> touched;
+        if (o.src_subset) {
+            touched.assign(g->comps.size(), 0);
+            for (size_t ci = 0; ci < g->comps.size(); ++ci) touched[ci] = !sub[ci].empty();
+        }
+        if ((rc = download_bc(ctx, g, bc, o.src_subset ? &touched : nullptr, bc_out))) return rc;
     }
     s->t_total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return BCX_OK;
