@@ -583,11 +583,29 @@ static TileJob make_job(bcx_ctx *ctx, const DComp &dc, const bcx_opts *o, double
 std::vector<int32_t> batched_sources(DComp &dc, const int32_t *subset, int64_t n_sub, int64_t *n_chain) {
     if (dc.rank.empty()) dc.rank = bfs_rank(dc.host);
     std::vector<int32_t> srcs;
-    if (subset) {
+    bool bucketed = false;
+    if (subset && n_sub * 8 >= dc.n) {
+        // a dense subset is put in rank order by bucketing (O(n), no sort:
+        // a step of the 10k-vertex workload spent 240 us sorting); a vertex
+        // the BFS did not reach (disconnected input) leaves it to the sort
+        std::vector<int32_t> at((size_t)dc.n, -1);
+        bucketed = true;
+        for (int64_t i = 0; i < n_sub && bucketed; ++i) {
+            const int32_t r = dc.rank[subset[i]];
+            if (r < 0) bucketed = false;
+            else at[(size_t)r] = subset[i];
+        }
+        if (bucketed) {
+            srcs.reserve((size_t)n_sub);
+            for (int32_t r = 0; r < dc.n; ++r)
+                if (at[(size_t)r] >= 0) srcs.push_back(at[(size_t)r]);
+        }
+    }
+    if (subset && !bucketed) {
         srcs.assign(subset, subset + n_sub);
         std::sort(srcs.begin(), srcs.end(), [&](int32_t a, int32_t b) { return dc.rank[a] < dc.rank[b]; });
         srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
-    } else {
+    } else if (!subset) {
         srcs.resize((size_t)dc.n);
         for (int32_t v = 0; v < dc.n; ++v) srcs[(size_t)dc.rank[v]] = v;
     }
