@@ -8,7 +8,7 @@
 namespace bcx {
 
 constexpr int SMALL_N = 2048;  // largest component handled by one workgroup
-// LDS of a small-kernel workgroup: 24 B + 4 B per vertex of the largest
+// LDS of a small-kernel workgroup: 32 B + 4 B per vertex of the largest
 // component, plus room for a tile's CSR, at most SMALL_LDS bytes (more
 // keeps the multi-launch kernels' blocks off the CU while it runs: 2%
 // slower on the 12.8M-vertex workload at 160 KiB)

@@ -4,8 +4,8 @@
 // uneven size share one launch instead of paying launches per level.
 //
 // Same method as the multi-launch batched kernels (bcx_batched.hip): lane k
-// of a wave64 owns source k; lane masks of the current / next level and the
-// visited set live in LDS (24 B per vertex), and so does a copy of the
+// of a wave64 owns source k; lane masks of three levels and the visited
+// set live in LDS (32 B per vertex), and so does a copy of the
 // tile's CSR when it fits beside them; sigma[v][64] and the dependency
 // terms t[v][64] live in a per-workgroup HBM slab; level lists Q / Qm are
 // built by a coalesced scan of the next-level masks.  sigma is pulled over
@@ -55,14 +55,14 @@ __device__ __forceinline__ uint64_t rl64(uint64_t x, int src) {
 
 // LDS bytes of a launch whose largest component has cap vertices, with csr
 // words for a copy of a tile's CSR (0: arcs stay in HBM)
-size_t small_lds_bytes(int64_t cap, int64_t csr) { return 24 * (size_t)cap + 4 * ((size_t)cap + 2 + (size_t)csr); }
+size_t small_lds_bytes(int64_t cap, int64_t csr) { return 32 * (size_t)cap + 4 * ((size_t)cap + 2 + (size_t)csr); }
 
 // Workgroup-shared state of a tile: LDS arrays, the tile's CSR (LDS or HBM)
 // and the workgroup's HBM slabs.
 template <typename TT>
 struct TileCtx {
-    uint64_t *vis, *mA, *mB;
-    int *qoff, *cnt, *ovf;
+    uint64_t *vis, *mA, *mB, *mC;
+    int *qoff, *cnt, *ovf;  // cnt: two list counters, used by alternate levels
     uint64_t *sig;
     TT *tt;
     int32_t *Q;
@@ -93,9 +93,11 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
         vis[v] = 0;
         x.mA[v] = 0;
         x.mB[v] = 0;
+        x.mC[v] = 0;
     }
     if (tid == 0) {
-        *x.cnt = 0;
+        x.cnt[0] = 0;
+        x.cnt[1] = 0;
         *x.ovf = 0;
         qoff[0] = 0;
         qoff[1] = tl.ns;
@@ -111,11 +113,16 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
         if (x.plv) x.plv[(int64_t)s * 64 + tid] = 0;
     }
     __syncthreads();
-    uint64_t *cur = x.mA, *nxt = x.mB;
+    // three mask arrays in rotation: level l's (cur), level l+1's being
+    // built (nxt, all zero at the start of the level) and level l-1's
+    // (old), which the level clears while it expands; the clearing then
+    // needs no phase and barrier of its own
+    uint64_t *cur = x.mA, *nxt = x.mB, *old = x.mC;
     int l = 0;
     // ----------------------------------------------------------- forward
     while (true) {
         const int q0 = qoff[l], q1 = qoff[l + 1];
+        for (int v = tid; v < n; v += SB) old[v] = 0;
         // expand: lane per entry, a whole wave for a larger entry
         for (int base = q0 + wv * 64; base < q1; base += SB) {
             const int e = base + lane;
@@ -154,7 +161,7 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
             const uint64_t bal = __ballot(N != 0);
             if (!bal) continue;
             int pos = 0;
-            if (lane == 0) pos = atomicAdd(x.cnt, __popcll(bal));
+            if (lane == 0) pos = atomicAdd(&x.cnt[l & 1], __popcll(bal));
             pos = rl(pos, 0) + __popcll(bal & (me - 1));
             if (N) {
                 Q[q1 + pos] = w;
@@ -162,11 +169,11 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
             }
         }
         __syncthreads();
-        const int q2 = q1 + *x.cnt;
-        __syncthreads();
+        // (the other counter, last read two levels ago, starts the next one)
+        const int q2 = q1 + x.cnt[l & 1];
         if (tid == 0) {
             qoff[l + 2] = q2;
-            *x.cnt = 0;
+            x.cnt[(l + 1) & 1] = 0;
         }
         if (q2 == q1) break;
         // pull: sigma of the new level over predecessors (CSR order)
@@ -224,10 +231,8 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
             if (lane == 0) vis[w] |= N;
         }
         __syncthreads();
-        // level l's masks are done with (the only non-zero words of cur)
-        for (int v = tid; v < n; v += SB) cur[v] = 0;
-        __syncthreads();
-        uint64_t *t = cur;
+        uint64_t *t = old;
+        old = cur;
         cur = nxt;
         nxt = t;
         l++;
@@ -236,13 +241,15 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
     __syncthreads();
     if (*x.ovf) return false;
     // ---------------------------------------------------------- backward
-    // cur still holds level D's masks: clear them so both arrays start at
-    // zero; each level then scatters its own masks for the level above
-    for (int v = tid; v < n; v += SB) cur[v] = 0;
-    __syncthreads();
-    uint64_t *succ = cur, *own = nxt;
+    // nxt (no level D+1) and old (cleared by level D) are zero, cur holds
+    // level D's masks.  The same rotation: the successors' masks (succ),
+    // this level's, scattered as its entries finish (own, zero at the start
+    // of the level), and the masks of two levels down (stale), cleared
+    // meanwhile
+    uint64_t *succ = nxt, *own = old, *stale = cur;
     for (int lb = D; lb >= 1; --lb) {
         const int q0 = qoff[lb], q1 = qoff[lb + 1];
+        for (int v = tid; v < n; v += SB) stale[v] = 0;
         for (int e = q0 + wv; e < q1; e += SW) {
             const int v = Q[e];
             const uint64_t M = Qm[e];
@@ -298,11 +305,8 @@ __device__ __forceinline__ bool small_tile(const SmallArgs &a, const SmallTile &
             if (lane == 0 && contrib != 0.0) gadd(glb(cp.bc) + v, contrib);
         }
         __syncthreads();
-        // level lb+1's masks (the only non-zero words of succ) are done
-        // with; level lb's become the successors
-        for (int v = tid; v < n; v += SB) succ[v] = 0;
-        __syncthreads();
-        uint64_t *t = succ;
+        uint64_t *t = stale;
+        stale = succ;
         succ = own;
         own = t;
     }
@@ -315,15 +319,16 @@ __global__ __launch_bounds__(SB) void k_small(SmallArgs a) {
     // launch's largest component (a.nmax vertices), then a.csr_cap words for
     // a copy of the tile's CSR (row pointers, then columns) when it fits
     extern __shared__ uint64_t s_dyn[];
-    __shared__ int s_item, s_cnt, s_ovf;
+    __shared__ int s_item, s_cnt[2], s_ovf;
     const int cap = (int)a.nmax;
     const int tid = threadIdx.x;
     TileCtx<TT> x;
     x.vis = s_dyn;
     x.mA = s_dyn + cap;
     x.mB = s_dyn + 2 * cap;
-    x.qoff = (int *)(s_dyn + 3 * cap);
-    x.cnt = &s_cnt;
+    x.mC = s_dyn + 3 * cap;
+    x.qoff = (int *)(s_dyn + 4 * cap);
+    x.cnt = s_cnt;
     x.ovf = &s_ovf;
     int *lrp = x.qoff + cap + 2;
     x.sig = a.sig + (int64_t)blockIdx.x * a.slab;
