@@ -27,7 +27,12 @@ namespace bcx {
 // as its waves are many: 16 waves per tile ran the 10k-vertex workload of 157 tiles in
 // 0.45x the time of 4 waves (8 waves: 0.67x).
 constexpr int SMALL_SB = 1024;
-constexpr int SUNR = 8;
+// Gathers per round of an entry's predecessor (successor) loop.  The
+// rounds are branch-free over their SUNR slots, and an entry of a small
+// component has one or two neighbours on the adjacent level for most of its
+// sources: 2 ran the 10k-vertex workload's launch in 0.75x the time of 8
+// (4: 0.81x, 1: 0.78x).
+constexpr int SUNR = 2;
 
 struct KahanS {
     float s = 0.f, c = 0.f;
