@@ -133,6 +133,7 @@ struct bcx_ctx {
 
 struct bcx_dgraph {
     std::vector<bcx::DComp> comps;
+    std::vector<int64_t> offs;  // comps[i].offset, packed for the source-slot search
     int64_t total_n = 0;
     bcx::DBuf pool;  // every component's CSR, reach and chain arrays, one allocation
 };

@@ -516,6 +516,7 @@ int bcx_upload(bcx_ctx *ctx, const bcx_component *comps, int32_t n_comps, bcx_dg
         }
     }
     g->total_n = off;
+    for (const DComp &dc : g->comps) g->offs.push_back(dc.offset);
     // every array of every component in one device allocation and one copy
     // (tens of thousands of small components would otherwise cost an
     // allocation and a copy per array); arrays start on 256-byte boundaries
@@ -673,13 +674,12 @@ int bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts, double *bc_out, b
     if (o.src_subset) {
         sub.assign(g->comps.size(), {});
         for (int64_t i = 0; i < o.n_src; ++i) {
-            int64_t slot = o.src_subset[i];
-            auto it = std::upper_bound(g->comps.begin(), g->comps.end(), slot,
-                                       [](int64_t v, const DComp &d) { return v < d.offset; });
-            if (it == g->comps.begin()) return ctx_fail(ctx, BCX_E_ARG, "source slot out of range");
-            --it;
-            if (slot - it->offset >= it->n) return ctx_fail(ctx, BCX_E_ARG, "source slot out of range");
-            sub[it - g->comps.begin()].push_back((int32_t)(slot - it->offset));
+            const int64_t slot = o.src_subset[i];
+            const size_t ci = (size_t)(std::upper_bound(g->offs.begin(), g->offs.end(), slot) - g->offs.begin());
+            if (ci == 0) return ctx_fail(ctx, BCX_E_ARG, "source slot out of range");
+            const DComp &dc = g->comps[ci - 1];
+            if (slot - dc.offset >= dc.n) return ctx_fail(ctx, BCX_E_ARG, "source slot out of range");
+            sub[ci - 1].push_back((int32_t)(slot - dc.offset));
         }
     }
     std::vector<SmallJob> small;  // batched path: components for the one-launch small kernel
