@@ -9,9 +9,10 @@ namespace bcx {
 
 constexpr int SMALL_N = 2048;  // largest component handled by one workgroup
 // LDS of a small-kernel workgroup: 32 B + 4 B per vertex of the largest
-// component, plus room for a tile's CSR, at most SMALL_LDS bytes (more
-// keeps the multi-launch kernels' blocks off the CU while it runs: 2%
-// slower on the 12.8M-vertex workload at 160 KiB)
+// component (72 KiB at SMALL_N), plus room for a tile's CSR as long as the
+// total stays within SMALL_LDS bytes (more keeps the multi-launch kernels'
+// blocks off the CU while it runs: 2% slower on the 12.8M-vertex workload
+// with room up to 160 KiB)
 constexpr size_t SMALL_LDS = 65536;
 size_t small_lds_bytes(int64_t cap, int64_t csr);
 int64_t small_csr_words(int64_t nmax, int64_t csr);
