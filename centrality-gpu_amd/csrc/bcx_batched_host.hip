@@ -1261,7 +1261,11 @@ void batched_sched_report(const DComp &dc, const bcx_opts *o, int32_t *bpc, int3
 // small components: one persistent launch over (component, 64-source) tiles
 // ------------------------------------------------------------------------
 struct SmallState {
-    DBuf comps, tiles, srcs, ws, counter, redo, sig, tt, Q, Qm, masks, qoffs, qr, ro, tag, lvl, dbg, psig;
+    // tables: a pass's components, tiles, sources and weights (one upload);
+    // tiles: the tiles of an FP64 rerun
+    DBuf tables, tiles, counter, redo, sig, tt, Q, Qm, masks, qoffs, qr, ro, tag, lvl, dbg, psig;
+    unsigned char *pin = nullptr;  // pinned host staging of the tables
+    size_t pin_bytes = 0;
     int grid = 0;
     // a pass started and not yet finished (persistent_start / _finish)
     bool pending = false, run_fp32 = false, run_graph = false;
@@ -1395,11 +1399,29 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
             }
         }
     }
+    // the four tables go up in one copy from pinned staging (the previous
+    // pass that used the staging has been waited for)
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_tiles = al(sizeof(SmallComp) * comps.size());
+    const size_t o_srcs = o_tiles + al(sizeof(SmallTile) * tiles.size());
+    const size_t o_ws = o_srcs + al(4 * srcs.size());
+    const size_t t_bytes = o_ws + al(8 * ws.size());
     int rc;
-    if ((rc = upload(ctx, comps.data(), sizeof(SmallComp) * comps.size(), ss.comps, "small comps")) ||
-        (rc = upload(ctx, tiles.data(), sizeof(SmallTile) * tiles.size(), ss.tiles, "small tiles")) ||
-        (rc = upload(ctx, srcs.data(), 4 * srcs.size(), ss.srcs, "small sources")) ||
-        (rc = upload(ctx, ws.data(), 8 * ws.size(), ss.ws, "small weights")) ||
+    if (ss.pin_bytes < t_bytes) {
+        if (ss.pin) (void)hipHostFree(ss.pin);
+        ss.pin = nullptr;
+        ss.pin_bytes = 0;
+        if ((rc = ctx_hip(ctx, hipHostMalloc((void **)&ss.pin, t_bytes, hipHostMallocDefault), "small tables staging")))
+            return rc;
+        ss.pin_bytes = t_bytes;
+    }
+    std::memcpy(ss.pin, comps.data(), sizeof(SmallComp) * comps.size());
+    std::memcpy(ss.pin + o_tiles, tiles.data(), sizeof(SmallTile) * tiles.size());
+    std::memcpy(ss.pin + o_srcs, srcs.data(), 4 * srcs.size());
+    std::memcpy(ss.pin + o_ws, ws.data(), 8 * ws.size());
+    if ((rc = ss.tables.ensure(ctx, t_bytes, "small tables")) ||
+        (rc = ctx_hip(ctx, hipMemcpyAsync(ss.tables.p, ss.pin, t_bytes, hipMemcpyHostToDevice, ctx->stream),
+                      "small tables")) ||
         (rc = ss.counter.ensure(ctx, 64, "small counters")) ||
         (rc = ss.redo.ensure(ctx, 4 * tiles.size(), "small redo list")))
         return rc;
@@ -1434,11 +1456,12 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
                   (rc = ss.dbg.ensure(ctx, 8 * slab * grid, "probe delta"))))
         return rc;
     SmallArgs a{};
-    a.comps = (const SmallComp *)ss.comps.p;
-    a.tiles = (const SmallTile *)ss.tiles.p;
+    unsigned char *tb = (unsigned char *)ss.tables.p;
+    a.comps = (const SmallComp *)tb;
+    a.tiles = (const SmallTile *)(tb + o_tiles);
     a.n_tiles = (int32_t)tiles.size();
-    a.srcs = (const int32_t *)ss.srcs.p;
-    a.ws = (const double *)ss.ws.p;
+    a.srcs = (const int32_t *)(tb + o_srcs);
+    a.ws = (const double *)(tb + o_ws);
     a.counter = (int32_t *)ss.counter.p;
     a.redo = (int32_t *)ss.redo.p;
     a.sig = (uint64_t *)ss.sig.p;
@@ -1544,8 +1567,9 @@ static void release_state(bcx_ctx *ctx, SmallState *&state) {
     if (!state) return;
     SmallState &ss = *state;
     if (ss.exec) (void)hipGraphExecDestroy(ss.exec);
-    for (DBuf *b : {&ss.comps, &ss.tiles, &ss.srcs, &ss.ws, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm,
-                    &ss.masks, &ss.qoffs, &ss.qr, &ss.ro, &ss.tag, &ss.lvl, &ss.dbg, &ss.psig})
+    if (ss.pin) (void)hipHostFree(ss.pin);
+    for (DBuf *b : {&ss.tables, &ss.tiles, &ss.counter, &ss.redo, &ss.sig, &ss.tt, &ss.Q, &ss.Qm, &ss.masks,
+                    &ss.qoffs, &ss.qr, &ss.ro, &ss.tag, &ss.lvl, &ss.dbg, &ss.psig})
         b->release(ctx);
     delete state;
     state = nullptr;
