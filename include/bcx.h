@@ -256,8 +256,10 @@ int  bcx_batched_order(bcx_ctx *ctx, bcx_dgraph *g, int32_t c, int32_t *order);
 /* Weighted Brandes over every component of g (begin_gpu free + active +
  * postprocess, Codes/begin_gpupgm.cu:42-89).  bc_out is a host array of
  * sum(n_c) doubles, component c at offset sum_{c'<c} n_{c'}; results are
- * ADDED to it.  Per-source weight (r(s)+1), per-target weight (r(t)+1)
- * (Codes/kernels_free.cu:391,409,425).                                      */
+ * ADDED to it (with a source subset, only the slices of components holding
+ * sources are touched; the others would only receive zeros).  bc_out may
+ * be NULL to leave the result on the device unread.  Per-source weight
+ * (r(s)+1), per-target weight (r(t)+1) (Codes/kernels_free.cu:391,409,425). */
 int  bcx_run(bcx_ctx *ctx, bcx_dgraph *g, const bcx_opts *opts,
              double *bc_out, bcx_stats *st);
 
