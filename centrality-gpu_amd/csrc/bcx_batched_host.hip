@@ -1411,16 +1411,27 @@ static int persistent_start(bcx_ctx *ctx, const std::vector<SmallJob> &jobs, con
         if (ss.pin) (void)hipHostFree(ss.pin);
         ss.pin = nullptr;
         ss.pin_bytes = 0;
-        if ((rc = ctx_hip(ctx, hipHostMalloc((void **)&ss.pin, t_bytes, hipHostMallocDefault), "small tables staging")))
-            return rc;
-        ss.pin_bytes = t_bytes;
+        if (hipHostMalloc((void **)&ss.pin, t_bytes, hipHostMallocDefault) == hipSuccess) {
+            ss.pin_bytes = t_bytes;
+        } else {
+            // no pinned memory (a low locked-memory limit): a pageable copy,
+            // which HIP stages before returning
+            (void)hipGetLastError();
+            ss.pin = nullptr;
+        }
     }
-    std::memcpy(ss.pin, comps.data(), sizeof(SmallComp) * comps.size());
-    std::memcpy(ss.pin + o_tiles, tiles.data(), sizeof(SmallTile) * tiles.size());
-    std::memcpy(ss.pin + o_srcs, srcs.data(), 4 * srcs.size());
-    std::memcpy(ss.pin + o_ws, ws.data(), 8 * ws.size());
+    std::vector<unsigned char> pageable;
+    unsigned char *stage = ss.pin;
+    if (!stage) {
+        pageable.resize(t_bytes);
+        stage = pageable.data();
+    }
+    std::memcpy(stage, comps.data(), sizeof(SmallComp) * comps.size());
+    std::memcpy(stage + o_tiles, tiles.data(), sizeof(SmallTile) * tiles.size());
+    std::memcpy(stage + o_srcs, srcs.data(), 4 * srcs.size());
+    std::memcpy(stage + o_ws, ws.data(), 8 * ws.size());
     if ((rc = ss.tables.ensure(ctx, t_bytes, "small tables")) ||
-        (rc = ctx_hip(ctx, hipMemcpyAsync(ss.tables.p, ss.pin, t_bytes, hipMemcpyHostToDevice, ctx->stream),
+        (rc = ctx_hip(ctx, hipMemcpyAsync(ss.tables.p, stage, t_bytes, hipMemcpyHostToDevice, ctx->stream),
                       "small tables")) ||
         (rc = ss.counter.ensure(ctx, 64, "small counters")) ||
         (rc = ss.redo.ensure(ctx, 4 * tiles.size(), "small redo list")))

@@ -618,9 +618,19 @@ static int download_bc(bcx_ctx *ctx, bcx_dgraph *g, const double *bc, const std:
         if (ctx->bc_pin) (void)hipHostFree(ctx->bc_pin);
         ctx->bc_pin = nullptr;
         ctx->bc_pin_bytes = 0;
-        int rc = ctx_hip(ctx, hipHostMalloc((void **)&ctx->bc_pin, bytes, hipHostMallocDefault), "result staging");
-        if (rc) return rc;
-        ctx->bc_pin_bytes = bytes;
+        if (hipHostMalloc((void **)&ctx->bc_pin, bytes, hipHostMallocDefault) == hipSuccess) {
+            ctx->bc_pin_bytes = bytes;
+        } else {
+            // no pinned memory (a low locked-memory limit): pageable staging
+            (void)hipGetLastError();
+            ctx->bc_pin = nullptr;
+        }
+    }
+    std::vector<double> pageable;
+    double *stage = ctx->bc_pin;
+    if (!stage) {
+        pageable.resize((size_t)packed);
+        stage = pageable.data();
     }
     const double *from = bc;
     int rc;
@@ -633,13 +643,12 @@ static int download_bc(bcx_ctx *ctx, bcx_dgraph *g, const double *bc, const std:
         if ((rc = ctx_hip(ctx, hipGetLastError(), "result gather"))) return rc;
         from = (const double *)ctx->bc_pack.p;
     }
-    if ((rc = ctx_hip(ctx, hipMemcpyAsync(ctx->bc_pin, from, bytes, hipMemcpyDeviceToHost, ctx->stream),
-                      "bc copy")) ||
+    if ((rc = ctx_hip(ctx, hipMemcpyAsync(stage, from, bytes, hipMemcpyDeviceToHost, ctx->stream), "bc copy")) ||
         (rc = ctx_hip(ctx, hipStreamSynchronize(ctx->stream), "bc copy")))
         return rc;
     for (const BcChunk &c : chunks) {
         double *o = bc_out + c.src;
-        const double *h = ctx->bc_pin + c.dst;
+        const double *h = stage + c.dst;
         for (int32_t i = 0; i < c.len; ++i) o[i] += h[i];
     }
     return BCX_OK;
